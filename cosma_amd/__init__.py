@@ -1,0 +1,113 @@
+"""cosma_amd — MI355X (gfx950) single-rank local-multiply path for COSMA.
+
+Python binding of the C ABI declared in include/cosma_amd.h, loaded with
+ctypes from the in-tree libcosma_amd.so.  Torch is used only to own device
+memory and streams in tests and benchmarks; every compute call goes through
+the HIP kernels of the library.  There is no CPU fallback: if the library is
+missing, importing `lib()` raises.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcosma_amd.so")
+
+OK = 0
+ERR_INVALID_ARG = 1
+ERR_HIP = 2
+ERR_ALLOC = 3
+ERR_UNSUPPORTED = 4
+ERR_NO_DEVICE = 5
+
+_lib = None
+
+c_i64 = ctypes.c_int64
+c_dbl = ctypes.c_double
+c_vp = ctypes.c_void_p
+c_char = ctypes.c_char
+
+
+class CosmaAmdError(RuntimeError):
+    """Raised when a C ABI call returns a non-zero status."""
+
+    def __init__(self, status, text):
+        super().__init__(f"cosma_amd status {status}: {text}")
+        self.status = status
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "cosma_amd_last_error": (ctypes.c_char_p, []),
+    "cosma_amd_version": (ctypes.c_char_p, []),
+    "cosma_amd_handle_create": (ctypes.c_int, [ctypes.POINTER(c_vp), ctypes.c_int, ctypes.c_int, c_i64, c_i64, c_i64]),
+    "cosma_amd_handle_destroy": (ctypes.c_int, [c_vp]),
+    "cosma_amd_dgemm": (ctypes.c_int, [c_vp, c_char, c_char, c_i64, c_i64, c_i64, c_dbl,
+                                       c_vp, c_i64, c_vp, c_i64, c_dbl, c_vp, c_i64, c_vp]),
+    "cosma_amd_block_transform": (ctypes.c_int, [c_vp, c_char, c_i64, c_i64, c_dbl, c_vp, c_i64,
+                                                 c_dbl, c_vp, c_i64, c_vp]),
+}
+
+
+def lib():
+    """Loads libcosma_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python -m cosma_amd.build` first")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status):
+    if status != OK:
+        raise CosmaAmdError(status, lib().cosma_amd_last_error().decode())
+    return status
+
+
+def _ch(t):
+    return t.encode() if isinstance(t, str) else bytes([t])
+
+
+class Handle:
+    """Owns a cosma_amd handle (streams + workspaces) on one device."""
+
+    def __init__(self, device=0, streams=0, tile_m=0, tile_n=0, tile_k=0):
+        h = c_vp()
+        check(lib().cosma_amd_handle_create(ctypes.byref(h), device, streams, tile_m, tile_n, tile_k))
+        self.ptr = h
+
+    def close(self):
+        if self.ptr:
+            lib().cosma_amd_handle_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # Raw pointer calls (device pointers as ints)
+    def dgemm(self, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream=None):
+        return check(lib().cosma_amd_dgemm(self.ptr, _ch(transa), _ch(transb), m, n, k, alpha,
+                                           a, lda, b, ldb, beta, c, ldc, stream))
+
+    def block_transform(self, op, rows, cols, alpha, src, ld_src, beta, dst, ld_dst, stream=None):
+        return check(lib().cosma_amd_block_transform(self.ptr, _ch(op), rows, cols, alpha, src, ld_src,
+                                                     beta, dst, ld_dst, stream))
+
+
+def colmajor(t):
+    """Returns (ptr, ld) of a torch tensor viewed as a column-major matrix.
+
+    A column-major r x c matrix is stored here as a torch tensor of shape
+    (c, ld) (row i of the tensor = column i of the matrix), so that
+    element (row, col) lives at ptr[row + col*ld].
+    """
+    assert t.dim() == 2 and t.stride(1) == 1
+    return t.data_ptr(), t.stride(0)

@@ -1,0 +1,134 @@
+// C ABI glue: argument validation, handle lifetime, error text.
+#include <cstdarg>
+#include <cstring>
+
+#include "common.hpp"
+
+namespace cosma_amd {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+void clear_error() { g_err[0] = '\0'; }
+
+int hip_fail(hipError_t e, const char *what, const char *file, int line) {
+    set_error("HIP error %d (%s) in %s at %s:%d", (int)e, hipGetErrorString(e), what, file, line);
+    return COSMA_AMD_ERR_HIP;
+}
+
+// The stream argument is used as given; NULL is the device's null stream,
+// as for a BLAS handle whose stream was never set.
+static hipStream_t pick_stream(cosma_amd_handle_t, void *stream) {
+    return static_cast<hipStream_t>(stream);
+}
+
+// Checks the usual BLAS leading-dimension rule ld >= max(1, rows).
+static bool ld_ok(int64_t ld, int64_t rows) { return ld >= (rows > 1 ? rows : 1); }
+
+}  // namespace cosma_amd
+
+using namespace cosma_amd;
+
+extern "C" {
+
+const char *cosma_amd_last_error(void) { return g_err; }
+
+const char *cosma_amd_version(void) { return "cosma_amd 0.1 gfx950"; }
+
+int cosma_amd_handle_create(cosma_amd_handle_t *handle, int device, int streams,
+                            int64_t tile_m, int64_t tile_n, int64_t tile_k) {
+    clear_error();
+    CA_CHECK(handle != nullptr, "handle_create: null handle pointer");
+    *handle = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        set_error("handle_create: no HIP device visible");
+        return COSMA_AMD_ERR_NO_DEVICE;
+    }
+    CA_CHECK(device >= 0 && device < count, "handle_create: device %d out of range [0,%d)", device, count);
+    hipDeviceProp_t prop;
+    CA_HIP(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        set_error("handle_create: device %d is %s, kernels are built for gfx950", device, prop.gcnArchName);
+        return COSMA_AMD_ERR_NO_DEVICE;
+    }
+    CA_HIP(hipSetDevice(device));
+    auto *h = new cosma_amd_handle_s();
+    h->device = device;
+    h->n_streams = streams > 0 ? streams : 2;
+    h->tile_m = tile_m > 0 ? tile_m : 5000;
+    h->tile_n = tile_n > 0 ? tile_n : 5000;
+    h->tile_k = tile_k > 0 ? tile_k : 5000;
+    h->n_cu = prop.multiProcessorCount;
+    for (int i = 0; i < h->n_streams; ++i) {
+        hipStream_t s;
+        hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            for (auto x : h->streams) hipStreamDestroy(x);
+            delete h;
+            return hip_fail(e, "hipStreamCreateWithFlags", __FILE__, __LINE__);
+        }
+        h->streams.push_back(s);
+    }
+    *handle = h;
+    return COSMA_AMD_OK;
+}
+
+int cosma_amd_handle_destroy(cosma_amd_handle_t h) {
+    clear_error();
+    if (!h) return COSMA_AMD_OK;
+    hipSetDevice(h->device);
+    for (auto s : h->streams) {
+        hipStreamSynchronize(s);
+        hipStreamDestroy(s);
+    }
+    delete h;
+    return COSMA_AMD_OK;
+}
+
+int cosma_amd_dgemm(cosma_amd_handle_t h, char transa, char transb,
+                    int64_t m, int64_t n, int64_t k, double alpha,
+                    const double *a, int64_t lda, const double *b, int64_t ldb,
+                    double beta, double *c, int64_t ldc, void *stream) {
+    clear_error();
+    CA_CHECK(h != nullptr, "dgemm: null handle");
+    const char ta = norm_trans(transa), tb = norm_trans(transb);
+    CA_CHECK(ta != 0, "dgemm: invalid transa '%c'", transa);
+    CA_CHECK(tb != 0, "dgemm: invalid transb '%c'", transb);
+    CA_CHECK(m >= 0 && n >= 0 && k >= 0, "dgemm: negative extent m=%lld n=%lld k=%lld",
+             (long long)m, (long long)n, (long long)k);
+    if (m == 0 || n == 0) return COSMA_AMD_OK;
+    CA_CHECK(ld_ok(ldc, m), "dgemm: ldc=%lld < m=%lld", (long long)ldc, (long long)m);
+    CA_CHECK(c != nullptr, "dgemm: null C");
+    if (k > 0 && alpha != 0.0) {
+        CA_CHECK(ld_ok(lda, ta == 'N' ? m : k), "dgemm: lda=%lld too small", (long long)lda);
+        CA_CHECK(ld_ok(ldb, tb == 'N' ? k : n), "dgemm: ldb=%lld too small", (long long)ldb);
+        CA_CHECK(a != nullptr && b != nullptr, "dgemm: null A or B");
+    }
+    CA_HIP(hipSetDevice(h->device));
+    return launch_dgemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, pick_stream(h, stream));
+}
+
+int cosma_amd_block_transform(cosma_amd_handle_t h, char op, int64_t rows, int64_t cols,
+                              double alpha, const double *src, int64_t ld_src, double beta,
+                              double *dst, int64_t ld_dst, void *stream) {
+    clear_error();
+    CA_CHECK(h != nullptr, "block_transform: null handle");
+    const char o = norm_trans(op);
+    CA_CHECK(o != 0, "block_transform: invalid op '%c'", op);
+    CA_CHECK(rows >= 0 && cols >= 0, "block_transform: negative extent");
+    if (rows == 0 || cols == 0) return COSMA_AMD_OK;
+    CA_CHECK(src != nullptr && dst != nullptr, "block_transform: null pointer");
+    CA_CHECK(ld_ok(ld_dst, rows), "block_transform: ld_dst=%lld < rows=%lld", (long long)ld_dst, (long long)rows);
+    CA_CHECK(ld_ok(ld_src, o == 'N' ? rows : cols), "block_transform: ld_src=%lld too small", (long long)ld_src);
+    CA_HIP(hipSetDevice(h->device));
+    return launch_block_transform(o, rows, cols, alpha, src, ld_src, beta, dst, ld_dst, pick_stream(h, stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,73 @@
+// Shared definitions for the gfx950 kernels and the C ABI glue.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/cosma_amd.h"
+
+namespace cosma_amd {
+
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// Thread-local error text backing cosma_amd_last_error().
+void set_error(const char *fmt, ...);
+void clear_error();
+
+// Records a HIP failure and returns COSMA_AMD_ERR_HIP.
+int hip_fail(hipError_t e, const char *what, const char *file, int line);
+
+#define CA_HIP(call)                                                      \
+    do {                                                                  \
+        hipError_t e_ = (call);                                           \
+        if (e_ != hipSuccess)                                             \
+            return ::cosma_amd::hip_fail(e_, #call, __FILE__, __LINE__);  \
+    } while (0)
+
+#define CA_CHECK(cond, ...)                                               \
+    do {                                                                  \
+        if (!(cond)) {                                                    \
+            ::cosma_amd::set_error(__VA_ARGS__);                          \
+            return COSMA_AMD_ERR_INVALID_ARG;                             \
+        }                                                                 \
+    } while (0)
+
+// Normalises a BLAS transpose character; returns 'N', 'T' or 0 if invalid.
+// 'C' (conjugate transpose) equals 'T' for real data.
+inline char norm_trans(char t) {
+    switch (t) {
+    case 'N': case 'n': return 'N';
+    case 'T': case 't': case 'C': case 'c': return 'T';
+    default: return 0;
+    }
+}
+
+}  // namespace cosma_amd
+
+// Handle: one device, a set of streams, and lazily grown workspaces.
+struct cosma_amd_handle_s {
+    int device = 0;
+    int n_streams = 2;
+    int64_t tile_m = 5000, tile_n = 5000, tile_k = 5000;
+    std::vector<hipStream_t> streams;
+    int n_cu = 256;
+};
+
+namespace cosma_amd {
+// Kernel launchers (stream already resolved, arguments already validated).
+int launch_dgemm(char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha,
+                 const double *a, int64_t lda, const double *b, int64_t ldb,
+                 double beta, double *c, int64_t ldc, hipStream_t s);
+int launch_scale(int64_t m, int64_t n, double beta, double *c, int64_t ldc,
+                 hipStream_t s);
+int launch_block_transform(char op, int64_t rows, int64_t cols, double alpha,
+                           const double *src, int64_t lds, double beta,
+                           double *dst, int64_t ldd, hipStream_t s);
+}  // namespace cosma_amd

@@ -1,0 +1,101 @@
+/*
+ * cosma_amd.h — C ABI of the MI355X (gfx950) local-multiply library.
+ *
+ * This is the thin layer between the C++ host code (cosma::local_multiply,
+ * the pxgemm front-end) and the hand-written HIP kernels.  Plain C types only:
+ * pointers, 64-bit extents, chars for transposes, int status codes.  Every
+ * matrix is column-major.  A status of COSMA_AMD_OK (0) means success; any
+ * other value is an error whose text is returned by cosma_amd_last_error().
+ *
+ * Each entry point names the reference interface it replaces (paths are
+ * relative to the COSMA v2.8.4 source tree; Tiled-MM and COSTA are external
+ * libraries whose interfaces are known from COSMA's call sites).
+ */
+#ifndef COSMA_AMD_H
+#define COSMA_AMD_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+enum {
+    COSMA_AMD_OK = 0,
+    COSMA_AMD_ERR_INVALID_ARG = 1,  /* bad transpose char, negative extent, ld too small */
+    COSMA_AMD_ERR_HIP = 2,          /* a HIP runtime call failed */
+    COSMA_AMD_ERR_ALLOC = 3,        /* device or pinned allocation failed */
+    COSMA_AMD_ERR_UNSUPPORTED = 4,  /* request outside what the kernels implement */
+    COSMA_AMD_ERR_NO_DEVICE = 5     /* no gfx950 device visible */
+};
+
+/* Text of the last error raised on the calling thread ("" if none). */
+const char *cosma_amd_last_error(void);
+
+/* Library version string, e.g. "cosma_amd 0.1 gfx950". */
+const char *cosma_amd_version(void);
+
+/* ---------------------------------------------------------------- handle */
+/*
+ * Opaque handle owning the HIP streams, the host-streaming staging ring and
+ * kernel workspace on one device.
+ * Replaces Tiled-MM's gpu::mm_handle<T> and gpu::make_context<T>(streams,
+ * tile_m, tile_n, tile_k) created by cosma_context
+ * (src/cosma/context.cpp:24 and :45; declared via context.hpp:59).
+ * streams <= 0 and tile_* <= 0 take the defaults of the reference
+ * (COSMA_GPU_STREAMS=2, COSMA_GPU_MAX_TILE_{M,N,K}=5000,
+ * src/cosma/environment_variables.hpp:57-62).
+ */
+typedef struct cosma_amd_handle_s *cosma_amd_handle_t;
+
+int cosma_amd_handle_create(cosma_amd_handle_t *handle, int device, int streams,
+                            int64_t tile_m, int64_t tile_n, int64_t tile_k);
+int cosma_amd_handle_destroy(cosma_amd_handle_t handle);
+
+/* ----------------------------------------------------- device-resident GEMM */
+/*
+ * C <- alpha * op(A) * op(B) + beta * C on device-resident operands, FP64,
+ * computed on the FP64 MFMA units (v_mfma_f64_16x16x4f64).
+ * op(A) is m x k, op(B) is k x n, C is m x n.  transa/transb in {'N','T','C'}
+ * (upper or lower case; 'C' equals 'T' for real data).
+ * Semantics kept from the reference:
+ *   m == 0 or n == 0  -> no-op;
+ *   k == 0 or alpha == 0 -> C <- beta * C   (src/cosma/multiply.cpp:99-106,
+ *                                            src/cosma/cosma_pxgemm.cpp:39-46);
+ *   beta == 0 -> C is never read (NaN in C does not propagate;
+ *                utils/pxgemm_utils.hpp:604-616).
+ * stream is a hipStream_t (NULL = the null stream).  The call is
+ * asynchronous with respect to the host.
+ * Replaces the per-tile vendor GEMM that Tiled-MM's gpu::gemm issues
+ * (call site src/cosma/local_multiply.cpp:245-260) and the host
+ * cosma::gemm -> cblas_dgemm (src/cosma/blas.cpp:24-48).
+ */
+int cosma_amd_dgemm(cosma_amd_handle_t handle, char transa, char transb,
+                    int64_t m, int64_t n, int64_t k, double alpha,
+                    const double *a, int64_t lda, const double *b, int64_t ldb,
+                    double beta, double *c, int64_t ldc, void *stream);
+
+/* ------------------------------------------------ local block transform */
+/*
+ * dst <- alpha * op(src) + beta * dst for one column-major block on the
+ * device; op in {'N','T','C'}; op(src) is rows x cols.  With alpha == 1 and
+ * beta == 0 this is a pure permutation and the result is bit-exact.
+ * beta == 0 never reads dst.
+ * Replaces COSTA's local block copy/transpose
+ * costa::memory::copy_and_transpose<T>(src, n_rows, n_cols, src_stride,
+ * dest, dest_stride, conj, workspace) (call site benchmarks/transpose.cpp:46-47)
+ * and the per-block alpha/beta update of costa::transformer<T>::schedule
+ * (call sites src/cosma/cosma_pxgemm.cpp:338-364, src/cosma/multiply.cpp:186-204).
+ * Here rows/cols describe op(src), i.e. dst is rows x cols.
+ */
+int cosma_amd_block_transform(cosma_amd_handle_t handle, char op,
+                              int64_t rows, int64_t cols, double alpha,
+                              const double *src, int64_t ld_src, double beta,
+                              double *dst, int64_t ld_dst, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* COSMA_AMD_H */
