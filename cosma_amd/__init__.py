@@ -43,6 +43,7 @@ SIGNATURES = {
     "cosma_amd_handle_destroy": (ctypes.c_int, [c_vp]),
     "cosma_amd_dgemm": (ctypes.c_int, [c_vp, c_char, c_char, c_i64, c_i64, c_i64, c_dbl,
                                        c_vp, c_i64, c_vp, c_i64, c_dbl, c_vp, c_i64, c_vp]),
+    "cosma_amd_dgemm_set_tuning": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "cosma_amd_block_transform": (ctypes.c_int, [c_vp, c_char, c_i64, c_i64, c_dbl, c_vp, c_i64,
                                                  c_dbl, c_vp, c_i64, c_vp]),
 }
@@ -100,6 +101,11 @@ class Handle:
     def block_transform(self, op, rows, cols, alpha, src, ld_src, beta, dst, ld_dst, stream=None):
         return check(lib().cosma_amd_block_transform(self.ptr, _ch(op), rows, cols, alpha, src, ld_src,
                                                      beta, dst, ld_dst, stream))
+
+
+def set_dgemm_tuning(tile_config=0, group_m=0):
+    """Selects the DGEMM tile shape (0 = automatic)."""
+    check(lib().cosma_amd_dgemm_set_tuning(tile_config, group_m))
 
 
 def colmajor(t):

@@ -115,6 +115,14 @@ int cosma_amd_dgemm(cosma_amd_handle_t h, char transa, char transb,
     return launch_dgemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, pick_stream(h, stream));
 }
 
+int cosma_amd_dgemm_set_tuning(int tile_config, int group_m) {
+    clear_error();
+    CA_CHECK(tile_config >= 0 && tile_config <= 3, "dgemm_set_tuning: tile_config %d not in 0..3", tile_config);
+    CA_CHECK(group_m >= 0, "dgemm_set_tuning: negative group_m");
+    set_dgemm_tuning(tile_config, group_m);
+    return COSMA_AMD_OK;
+}
+
 int cosma_amd_block_transform(cosma_amd_handle_t h, char op, int64_t rows, int64_t cols,
                               double alpha, const double *src, int64_t ld_src, double beta,
                               double *dst, int64_t ld_dst, void *stream) {

@@ -65,6 +65,10 @@ namespace cosma_amd {
 int launch_dgemm(char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha,
                  const double *a, int64_t lda, const double *b, int64_t ldb,
                  double beta, double *c, int64_t ldc, hipStream_t s);
+int launch_dgemm_cfg(int cfg, int group_m, char ta, char tb, int64_t m, int64_t n, int64_t k,
+                     double alpha, const double *a, int64_t lda, const double *b, int64_t ldb,
+                     double beta, double *c, int64_t ldc, hipStream_t s);
+void set_dgemm_tuning(int cfg, int group_m);
 int launch_scale(int64_t m, int64_t n, double beta, double *c, int64_t ldc,
                  hipStream_t s);
 int launch_block_transform(char op, int64_t rows, int64_t cols, double alpha,

@@ -7,13 +7,17 @@
 // or cosma::gemm -> cblas_dgemm at src/cosma/blas.cpp:24-48).
 //
 // Work decomposition
-//   * A 256-thread workgroup (4 waves of 64 lanes) owns a BM x BN = 128 x 128
-//     block of C; each wave owns a 64 x 64 quarter, i.e. 4 x 4 MFMA blocks of
-//     16 x 16, held as 16 accumulators of 4 doubles per lane.
+//   * A 256-thread workgroup (4 waves of 64 lanes, 2 x 2) owns a BM x BN block
+//     of C; each wave owns a (BM/2) x (BN/2) quarter made of FM x FN MFMA
+//     blocks of 16 x 16, each held as 4 doubles per lane.
 //   * K is walked in steps of BK = 16.  Both operand tiles are staged in LDS
 //     in a "k-pair" layout  tile[kp][row] = (x[row][2kp], x[row][2kp+1])  so
 //     that one 16-byte LDS read per lane feeds two v_mfma_f64_16x16x4f64
 //     (lane group g = lane/16 supplies k = 2kp and 2kp+1 of the same pair).
+//     Each kp-row of the tile is padded by one 16-byte entry so that the
+//     k-contiguous global-load pattern (8 lanes per matrix row) writes LDS
+//     without bank conflicts, while the MFMA reads (16 lanes on 16
+//     consecutive rows of one kp) stay conflict-free too.
 //     The order in which k is summed is a permutation of 0..K-1; it only
 //     changes rounding, not the mathematical result.
 //   * LDS is double-buffered: the global loads of step t+1 are issued into
@@ -23,6 +27,11 @@
 //     operands whose k is contiguous (A with 'T', B with 'N') use two
 //     different global-load patterns that land in the same LDS layout; all
 //     four transpose combinations share the MFMA loop.
+//   * Tile order is XCD-aware: the hardware hands workgroup b to XCD b % 8,
+//     so workgroup ids are remapped so that each XCD receives a contiguous
+//     range of logical tiles, and logical tiles walk GROUP_M tile-rows before
+//     moving right.  The ~32 tiles resident on one XCD then form a compact
+//     block of C whose A and B panels are shared through that XCD's L2.
 //
 // MFMA v_mfma_f64_16x16x4f64 operand layout (lane l, g = l/16, i = l%16),
 // decoded on gfx950 hardware:
@@ -37,165 +46,234 @@
 namespace cosma_amd {
 namespace {
 
-constexpr int BM = 128;
-constexpr int BN = 128;
 constexpr int BK = 16;
 constexpr int KP = BK / 2;   // k-pairs per step
 constexpr int NT = 256;      // threads per workgroup
-constexpr int GROUP_M = 8;   // tile-rows per rasterisation group
+constexpr int NXCD = 8;      // XCDs on MI355X
 
-struct alignas(16) Smem {
-    f64x2 a[2][KP][BM];
-    f64x2 b[2][KP][BN];
-};
-
-// Loads of one operand tile into 4 f64x2 registers per thread.
+// Loads of one operand tile (ROWS x BK) into registers.
 //   RowContig: element (r, k) at x[r + k * ld]   (A 'N', B 'T')
+//     work item idx -> row pair rp = idx % (ROWS/2), k-pair kp = idx / (ROWS/2);
+//     two loads (k = 2kp and 2kp+1), each covering rows 2rp, 2rp+1.
 //   otherwise: element (r, k) at x[k + r * ld]   (A 'T', B 'N')
-// r0/k0: tile origin; R/K: operand extents (rows of op(X), shared dim).
-template <bool RowContig>
-__device__ __forceinline__ void load_tile(const double *__restrict__ x, int64_t ld,
-                                          int64_t r0, int64_t k0, int64_t R, int64_t K,
-                                          bool vec_ok, f64x2 (&v)[4]) {
-    const int t = threadIdx.x;
-    const bool full = vec_ok && (r0 + BM <= R) && (k0 + BK <= K);
-    if (RowContig) {
-        // thread -> rows r = r0 + 2*(t%64) (+1), k-pairs kp = t/64 + 4q, q = 0,1
-        const int64_t r = r0 + 2 * (t & 63);
+//     work item idx -> k-pair kp = idx % KP, row r = idx / KP; one load.
+// Rows at or beyond R are clamped to a valid row: they only feed rows (or
+// columns) of C that the epilogue never stores, so their values are
+// irrelevant and no branch is needed.  Only k must be exact: full steps
+// (load) need no k check; the one partial step at the end of K (load_tail)
+// zero-fills k >= K.
+// VEC: 16-byte loads; requires 16-byte aligned x, even ld and, for
+// RowContig, even R (so a row pair never straddles the end of a column).
+template <bool RowContig, int ROWS, bool VEC>
+struct TileIO {
+    static constexpr int ITEMS = RowContig ? (ROWS / 2) * KP : ROWS * KP;
+    static constexpr int PER_T = ITEMS / NT;           // work items per thread
+    static constexpr int NREG = RowContig ? 2 * PER_T : PER_T;
+    static_assert(ITEMS % NT == 0, "tile must divide evenly over the workgroup");
+
+    __device__ __forceinline__ static void load(const double *__restrict__ x, int64_t ld,
+                                                int64_t r0, int64_t k0, int64_t R,
+                                                f64x2 (&v)[NREG]) {
+        const int t = threadIdx.x;
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const int64_t k = k0 + 2 * ((t >> 6) + 4 * q);
-            if (full) {
-                v[2 * q] = *reinterpret_cast<const f64x2 *>(x + r + k * ld);
-                v[2 * q + 1] = *reinterpret_cast<const f64x2 *>(x + r + (k + 1) * ld);
+        for (int q = 0; q < PER_T; ++q) {
+            const int idx = t + NT * q;
+            if (RowContig) {
+                const int64_t k = k0 + 2 * (idx / (ROWS / 2));
+                if (VEC) {
+                    const int64_t r = min(r0 + 2 * (idx % (ROWS / 2)), R - 2);
+                    v[2 * q] = *reinterpret_cast<const f64x2 *>(x + r + k * ld);
+                    v[2 * q + 1] = *reinterpret_cast<const f64x2 *>(x + r + (k + 1) * ld);
+                } else {
+                    const int64_t ra = min(r0 + 2 * (idx % (ROWS / 2)), R - 1);
+                    const int64_t rb = min(r0 + 2 * (idx % (ROWS / 2)) + 1, R - 1);
+                    v[2 * q].x = x[ra + k * ld];
+                    v[2 * q].y = x[rb + k * ld];
+                    v[2 * q + 1].x = x[ra + (k + 1) * ld];
+                    v[2 * q + 1].y = x[rb + (k + 1) * ld];
+                }
             } else {
-#pragma unroll
-                for (int dk = 0; dk < 2; ++dk) {
-                    const bool kin = (k + dk) < K;
-                    v[2 * q + dk].x = (kin && r < R) ? x[r + (k + dk) * ld] : 0.0;
-                    v[2 * q + dk].y = (kin && r + 1 < R) ? x[r + 1 + (k + dk) * ld] : 0.0;
+                const int64_t k = k0 + 2 * (idx % KP);
+                const int64_t r = min(r0 + idx / KP, R - 1);
+                if (VEC) {
+                    v[q] = *reinterpret_cast<const f64x2 *>(x + k + r * ld);
+                } else {
+                    v[q].x = x[k + r * ld];
+                    v[q].y = x[k + 1 + r * ld];
                 }
             }
         }
-    } else {
-        // thread -> k-pair kp = t%8, rows r = r0 + t/8 + 32q, q = 0..3
-        const int64_t k = k0 + 2 * (t & 7);
+    }
+
+    __device__ __forceinline__ static void load_tail(const double *__restrict__ x, int64_t ld,
+                                                     int64_t r0, int64_t k0, int64_t R, int64_t K,
+                                                     f64x2 (&v)[NREG]) {
+        const int t = threadIdx.x;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int64_t r = r0 + (t >> 3) + 32 * q;
-            if (full) {
-                v[q] = *reinterpret_cast<const f64x2 *>(x + k + r * ld);
+        for (int q = 0; q < PER_T; ++q) {
+            const int idx = t + NT * q;
+            if (RowContig) {
+                const int64_t ra = min(r0 + 2 * (idx % (ROWS / 2)), R - 1);
+                const int64_t rb = min(r0 + 2 * (idx % (ROWS / 2)) + 1, R - 1);
+                const int64_t k = k0 + 2 * (idx / (ROWS / 2));
+#pragma unroll
+                for (int dk = 0; dk < 2; ++dk) {
+                    const bool kin = (k + dk) < K;
+                    v[2 * q + dk].x = kin ? x[ra + (k + dk) * ld] : 0.0;
+                    v[2 * q + dk].y = kin ? x[rb + (k + dk) * ld] : 0.0;
+                }
             } else {
-                const bool rin = r < R;
-                v[q].x = (rin && k < K) ? x[k + r * ld] : 0.0;
-                v[q].y = (rin && k + 1 < K) ? x[k + 1 + r * ld] : 0.0;
+                const int64_t k = k0 + 2 * (idx % KP);
+                const int64_t r = min(r0 + idx / KP, R - 1);
+                v[q].x = (k < K) ? x[k + r * ld] : 0.0;
+                v[q].y = (k + 1 < K) ? x[k + 1 + r * ld] : 0.0;
             }
         }
     }
-}
 
-template <bool RowContig, int ROWS>
-__device__ __forceinline__ void store_tile(f64x2 (*tile)[ROWS], const f64x2 (&v)[4]) {
-    const int t = threadIdx.x;
-    if (RowContig) {
-        const int r = 2 * (t & 63);
+    __device__ __forceinline__ static void store(f64x2 (*tile)[ROWS + 1], const f64x2 (&v)[NREG]) {
+        const int t = threadIdx.x;
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const int kp = (t >> 6) + 4 * q;
-            f64x2 lo, hi;
-            lo.x = v[2 * q].x;  lo.y = v[2 * q + 1].x;   // row r:   (k, k+1)
-            hi.x = v[2 * q].y;  hi.y = v[2 * q + 1].y;   // row r+1: (k, k+1)
-            tile[kp][r] = lo;
-            tile[kp][r + 1] = hi;
+        for (int q = 0; q < PER_T; ++q) {
+            const int idx = t + NT * q;
+            if (RowContig) {
+                const int r = 2 * (idx % (ROWS / 2));
+                const int kp = idx / (ROWS / 2);
+                f64x2 lo, hi;
+                lo.x = v[2 * q].x;  lo.y = v[2 * q + 1].x;   // row r:   (k, k+1)
+                hi.x = v[2 * q].y;  hi.y = v[2 * q + 1].y;   // row r+1: (k, k+1)
+                tile[kp][r] = lo;
+                tile[kp][r + 1] = hi;
+            } else {
+                tile[idx % KP][idx / KP] = v[q];
+            }
         }
-    } else {
-        const int kp = t & 7;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) tile[kp][(t >> 3) + 32 * q] = v[q];
     }
+};
+
+// Maps a hardware workgroup id to a logical tile id so that XCD x (which
+// receives ids b with b % 8 == x) owns the contiguous logical range
+// [start_x, start_x + count_x).  Bijective on [0, T).
+__device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t T) {
+    const int64_t xcd = b % NXCD, local = b / NXCD;
+    const int64_t q = T / NXCD, r = T % NXCD;
+    return (xcd < r) ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
 }
 
-template <bool TA, bool TB>
+template <int BM, int BN, bool TA, bool TB, bool VEC>
 __global__ __launch_bounds__(NT) void dgemm_mfma_kernel(
     int64_t M, int64_t N, int64_t K, double alpha,
     const double *__restrict__ A, int64_t lda,
     const double *__restrict__ B, int64_t ldb,
     double beta, double *__restrict__ C, int64_t ldc,
-    int vec_a, int vec_b, int vec_c) {
-    __shared__ Smem sm;
+    int vec_c, int group_m) {
+    constexpr int FM = BM / 32, FN = BN / 32;    // MFMA blocks per wave
+    using IOA = TileIO<!TA, BM, VEC>;
+    using IOB = TileIO<TB, BN, VEC>;
+    __shared__ f64x2 sa[2][KP][BM + 1];
+    __shared__ f64x2 sb[2][KP][BN + 1];
 
-    // ---- tile rasterisation: groups of GROUP_M tile-rows walk along n
+    // ---- tile rasterisation (XCD-aware, grouped along m)
     const int64_t tiles_m = (M + BM - 1) / BM;
     const int64_t tiles_n = (N + BN - 1) / BN;
-    const int64_t bid = blockIdx.x;
-    const int64_t per_group = (int64_t)GROUP_M * tiles_n;
-    const int64_t group = bid / per_group;
-    const int64_t first_m = group * GROUP_M;
-    const int64_t gsize = min((int64_t)GROUP_M, tiles_m - first_m);
-    const int64_t in_group = bid - group * per_group;
-    const int64_t tm = first_m + in_group % gsize;
-    const int64_t tn = in_group / gsize;
-    const int64_t m0 = tm * BM, n0 = tn * BN;
+    const int64_t bid = xcd_remap(blockIdx.x, tiles_m * tiles_n);
+    const int64_t per_group = (int64_t)group_m * tiles_n;
+    const int64_t grp = bid / per_group;
+    const int64_t first_m = grp * group_m;
+    const int64_t gsize = min((int64_t)group_m, tiles_m - first_m);
+    const int64_t in_group = bid - grp * per_group;
+    const int64_t m0 = (first_m + in_group % gsize) * BM;
+    const int64_t n0 = (in_group / gsize) * BN;
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     const int g = lane >> 4, li = lane & 15;
     const int arow = 4 * (li & 3) + (li >> 2);   // row permutation, see header
+    const int abase = wm * (BM / 2) + arow;
+    const int bbase = wn * (BN / 2) + li;
 
-    f64x4 acc[4][4];
+    // The accumulators are zeroed by an MFMA (0 * 0 + 0) rather than by
+    // VALU moves: values that originate from an MFMA are allocated to AGPRs,
+    // which keeps them there across the k loop; zeroing with moves led the
+    // compiler to carry them in VGPRs and copy all of them to and from AGPRs
+    // in every iteration.
+    f64x4 acc[FM][FN];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+        for (int j = 0; j < FN; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(0.0, 0.0, f64x4{0.0, 0.0, 0.0, 0.0}, 0, 0, 0);
 
+    // Fragment registers: F[h] holds the A/B fragments of k-sub-step h (two
+    // MFMA k-slices each) so that LDS reads of one sub-step overlap the
+    // MFMAs of the other.
+    f64x2 af[2][FM], bf[2][FN];
+    auto read_frags = [&](int st, int ks, int h) {
+        const int kp = 4 * ks + g;
+#pragma unroll
+        for (int i = 0; i < FM; ++i) af[h][i] = sa[st][kp][abase + 16 * i];
+#pragma unroll
+        for (int j = 0; j < FN; ++j) bf[h][j] = sb[st][kp][bbase + 16 * j];
+    };
+    auto mfma_frags = [&](int h) {
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[h][i].x, bf[h][j].x, acc[i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[h][i].y, bf[h][j].y, acc[i][j], 0, 0, 0);
+    };
+
+    const int64_t nk_full = K / BK;
     const int64_t nk = (K + BK - 1) / BK;
-    f64x2 ra[4], rb[4];
+    f64x2 ra[IOA::NREG], rb[IOB::NREG];
+    auto load_step = [&](int64_t kn) {
+        if (kn < nk_full) {
+            IOA::load(A, lda, m0, kn * BK, M, ra);
+            IOB::load(B, ldb, n0, kn * BK, N, rb);
+        } else {                     // the partial step at the end of K
+            IOA::load_tail(A, lda, m0, kn * BK, M, K, ra);
+            IOB::load_tail(B, ldb, n0, kn * BK, N, K, rb);
+        }
+    };
 
-    load_tile<!TA>(A, lda, m0, 0, M, K, vec_a, ra);
-    load_tile<TB>(B, ldb, n0, 0, N, K, vec_b, rb);
-    store_tile<!TA, BM>(sm.a[0], ra);
-    store_tile<TB, BN>(sm.b[0], rb);
+    // Pipeline: registers hold step kt+1 while step kt is multiplied out of
+    // LDS stage kt&1; halfway through step kt they are written to stage
+    // (kt+1)&1 and refilled with step kt+2 from global memory.
+    load_step(0);
+    IOA::store(sa[0], ra);
+    IOB::store(sb[0], rb);
+    if (nk > 1) load_step(1);
     __syncthreads();
+    read_frags(0, 0, 0);
 
     for (int64_t kt = 0; kt < nk; ++kt) {
         const int st = kt & 1;
-        const bool more = kt + 1 < nk;
-        if (more) {
-            load_tile<!TA>(A, lda, m0, (kt + 1) * BK, M, K, vec_a, ra);
-            load_tile<TB>(B, ldb, n0, (kt + 1) * BK, N, K, vec_b, rb);
+        read_frags(st, 1, 1);
+        mfma_frags(0);
+        if (kt + 1 < nk) {
+            IOA::store(sa[st ^ 1], ra);
+            IOB::store(sb[st ^ 1], rb);
+            if (kt + 2 < nk) load_step(kt + 2);
         }
-#pragma unroll
-        for (int ks = 0; ks < BK / 8; ++ks) {
-            const int kp = 4 * ks + g;
-            f64x2 af[4], bf[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) af[i] = sm.a[st][kp][wm * 64 + 16 * i + arow];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) bf[j] = sm.b[st][kp][wn * 64 + 16 * j + li];
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i].x, bf[j].x, acc[i][j], 0, 0, 0);
-                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i].y, bf[j].y, acc[i][j], 0, 0, 0);
-                }
-        }
-        if (more) {
-            store_tile<!TA, BM>(sm.a[st ^ 1], ra);
-            store_tile<TB, BN>(sm.b[st ^ 1], rb);
-        }
+        mfma_frags(1);
         __syncthreads();
+        if (kt + 1 < nk) read_frags(st ^ 1, 0, 0);
     }
 
     // ---- epilogue: lane holds rows 4g..4g+3 (contiguous) of column li
     const bool beta0 = (beta == 0.0);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int64_t row = m0 + wm * 64 + 16 * i + 4 * g;
+    for (int i = 0; i < FM; ++i) {
+        const int64_t row = m0 + wm * (BM / 2) + 16 * i + 4 * g;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int64_t col = n0 + wn * 64 + 16 * j + li;
+        for (int j = 0; j < FN; ++j) {
+            const int64_t col = n0 + wn * (BN / 2) + 16 * j + li;
             if (col >= N) continue;
             double *cp = C + row + col * ldc;
             const f64x4 r = acc[i][j];
@@ -237,6 +315,41 @@ __global__ void scale_kernel(int64_t M, int64_t N, double beta, double *__restri
 
 inline bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
 
+template <int BM, int BN>
+int launch_cfg(char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha,
+               const double *a, int64_t lda, const double *b, int64_t ldb,
+               double beta, double *c, int64_t ldc, int group_m, hipStream_t s) {
+    const int64_t tiles = ((m + BM - 1) / BM) * ((n + BN - 1) / BN);
+    if (tiles > 0x7fffffffLL) {
+        set_error("dgemm: %lld output tiles exceed the grid limit", (long long)tiles);
+        return COSMA_AMD_ERR_UNSUPPORTED;
+    }
+    const bool TA = ta == 'T', TB = tb == 'T';
+    // 16-byte operand loads need aligned pointers, even leading dimensions
+    // and, for row-contiguous operands, an even row count.
+    const bool vec = aligned16(a) && (lda % 2 == 0) && aligned16(b) && (ldb % 2 == 0) &&
+                     (TA || m % 2 == 0) && (!TB || n % 2 == 0);
+    const int vec_c = aligned16(c) && (ldc % 2 == 0);
+    const dim3 grid((unsigned)tiles), block(NT);
+#define CA_LAUNCH(X, Y, V)                                                                        \
+    hipLaunchKernelGGL((dgemm_mfma_kernel<BM, BN, X, Y, V>), grid, block, 0, s, m, n, k, alpha, a, \
+                       lda, b, ldb, beta, c, ldc, vec_c, group_m)
+    if (vec) {
+        if (!TA && !TB) CA_LAUNCH(false, false, true);
+        else if (TA && !TB) CA_LAUNCH(true, false, true);
+        else if (!TA && TB) CA_LAUNCH(false, true, true);
+        else CA_LAUNCH(true, true, true);
+    } else {
+        if (!TA && !TB) CA_LAUNCH(false, false, false);
+        else if (TA && !TB) CA_LAUNCH(true, false, false);
+        else if (!TA && TB) CA_LAUNCH(false, true, false);
+        else CA_LAUNCH(true, true, false);
+    }
+#undef CA_LAUNCH
+    CA_HIP(hipGetLastError());
+    return COSMA_AMD_OK;
+}
+
 }  // namespace
 
 int launch_scale(int64_t m, int64_t n, double beta, double *c, int64_t ldc, hipStream_t s) {
@@ -249,31 +362,40 @@ int launch_scale(int64_t m, int64_t n, double beta, double *c, int64_t ldc, hipS
     return COSMA_AMD_OK;
 }
 
+// Tile configuration: 0 = automatic, 1 = 128x128, 2 = 256x128, 3 = 128x256.
+static int g_cfg_override = 0;
+static int g_group_override = 0;
+
+int launch_dgemm_cfg(int cfg, int group_m, char ta, char tb, int64_t m, int64_t n, int64_t k,
+                     double alpha, const double *a, int64_t lda, const double *b, int64_t ldb,
+                     double beta, double *c, int64_t ldc, hipStream_t s) {
+    if (m == 0 || n == 0) return COSMA_AMD_OK;
+    if (k == 0 || alpha == 0.0) return launch_scale(m, n, beta, c, ldc, s);
+    if (cfg == 0) {
+        // Large tiles need enough of them to fill 256 CUs.
+        const int64_t t256 = ((m + 255) / 256) * ((n + 127) / 128);
+        cfg = (t256 >= 512) ? 2 : 1;
+    }
+    switch (cfg) {
+    case 1: return launch_cfg<128, 128>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, group_m > 0 ? group_m : 8, s);
+    case 2: return launch_cfg<256, 128>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, group_m > 0 ? group_m : 4, s);
+    case 3: return launch_cfg<128, 256>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, group_m > 0 ? group_m : 8, s);
+    default:
+        set_error("dgemm: unknown tile configuration %d", cfg);
+        return COSMA_AMD_ERR_INVALID_ARG;
+    }
+}
+
 int launch_dgemm(char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha,
                  const double *a, int64_t lda, const double *b, int64_t ldb,
                  double beta, double *c, int64_t ldc, hipStream_t s) {
-    if (m == 0 || n == 0) return COSMA_AMD_OK;
-    if (k == 0 || alpha == 0.0) return launch_scale(m, n, beta, c, ldc, s);
-    const int64_t tiles = ((m + BM - 1) / BM) * ((n + BN - 1) / BN);
-    if (tiles > 0x7fffffffLL) {
-        set_error("dgemm: %lld output tiles exceed the grid limit", (long long)tiles);
-        return COSMA_AMD_ERR_UNSUPPORTED;
-    }
-    const int vec_a = aligned16(a) && (lda % 2 == 0);
-    const int vec_b = aligned16(b) && (ldb % 2 == 0);
-    const int vec_c = aligned16(c) && (ldc % 2 == 0);
-    const dim3 grid((unsigned)tiles), block(NT);
-    const bool TA = ta == 'T', TB = tb == 'T';
-    if (!TA && !TB)
-        hipLaunchKernelGGL((dgemm_mfma_kernel<false, false>), grid, block, 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, vec_a, vec_b, vec_c);
-    else if (TA && !TB)
-        hipLaunchKernelGGL((dgemm_mfma_kernel<true, false>), grid, block, 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, vec_a, vec_b, vec_c);
-    else if (!TA && TB)
-        hipLaunchKernelGGL((dgemm_mfma_kernel<false, true>), grid, block, 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, vec_a, vec_b, vec_c);
-    else
-        hipLaunchKernelGGL((dgemm_mfma_kernel<true, true>), grid, block, 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, vec_a, vec_b, vec_c);
-    CA_HIP(hipGetLastError());
-    return COSMA_AMD_OK;
+    return launch_dgemm_cfg(g_cfg_override, g_group_override, ta, tb, m, n, k, alpha, a, lda, b,
+                            ldb, beta, c, ldc, s);
+}
+
+void set_dgemm_tuning(int cfg, int group_m) {
+    g_cfg_override = cfg;
+    g_group_override = group_m;
 }
 
 }  // namespace cosma_amd

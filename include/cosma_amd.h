@@ -77,6 +77,16 @@ int cosma_amd_dgemm(cosma_amd_handle_t handle, char transa, char transb,
                     const double *a, int64_t lda, const double *b, int64_t ldb,
                     double beta, double *c, int64_t ldc, void *stream);
 
+/*
+ * Selects the DGEMM tile shape for later cosma_amd_dgemm calls (process-wide):
+ * tile_config 0 = automatic, 1 = 128x128, 2 = 256x128, 3 = 128x256 (m x n of
+ * C per workgroup); group_m = tile-rows walked before moving along n
+ * (0 = default for the shape).  No counterpart in the reference (Tiled-MM's
+ * tile sizes, COSMA_GPU_MAX_TILE_*, describe host streaming tiles, not
+ * kernel tiles); used by the tuner and the benchmarks.
+ */
+int cosma_amd_dgemm_set_tuning(int tile_config, int group_m);
+
 /* ------------------------------------------------ local block transform */
 /*
  * dst <- alpha * op(src) + beta * dst for one column-major block on the
