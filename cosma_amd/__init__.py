@@ -43,6 +43,7 @@ SIGNATURES = {
     "cosma_amd_handle_destroy": (ctypes.c_int, [c_vp]),
     "cosma_amd_dgemm": (ctypes.c_int, [c_vp, c_char, c_char, c_i64, c_i64, c_i64, c_dbl,
                                        c_vp, c_i64, c_vp, c_i64, c_dbl, c_vp, c_i64, c_vp]),
+    "cosma_amd_fill_uniform": (ctypes.c_int, [c_vp, c_i64, c_i64, c_i64, ctypes.c_uint64, ctypes.c_uint64, c_vp]),
     "cosma_amd_dgemm_set_tuning": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "cosma_amd_block_transform": (ctypes.c_int, [c_vp, c_char, c_i64, c_i64, c_dbl, c_vp, c_i64,
                                                  c_dbl, c_vp, c_i64, c_vp]),
@@ -101,6 +102,11 @@ class Handle:
     def block_transform(self, op, rows, cols, alpha, src, ld_src, beta, dst, ld_dst, stream=None):
         return check(lib().cosma_amd_block_transform(self.ptr, _ch(op), rows, cols, alpha, src, ld_src,
                                                      beta, dst, ld_dst, stream))
+
+
+def fill_uniform(ptr, rows, cols, ld, seed, offset=0, stream=None):
+    """Seeded uniform [-1, 1) fill of a column-major device matrix."""
+    check(lib().cosma_amd_fill_uniform(ptr, rows, cols, ld, seed, offset, stream))
 
 
 def set_dgemm_tuning(tile_config=0, group_m=0):

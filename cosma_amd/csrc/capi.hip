@@ -139,4 +139,14 @@ int cosma_amd_block_transform(cosma_amd_handle_t h, char op, int64_t rows, int64
     return launch_block_transform(o, rows, cols, alpha, src, ld_src, beta, dst, ld_dst, pick_stream(h, stream));
 }
 
+int cosma_amd_fill_uniform(double *x, int64_t rows, int64_t cols, int64_t ld, uint64_t seed,
+                           uint64_t offset, void *stream) {
+    clear_error();
+    CA_CHECK(rows >= 0 && cols >= 0, "fill_uniform: negative extent");
+    if (rows == 0 || cols == 0) return COSMA_AMD_OK;
+    CA_CHECK(x != nullptr, "fill_uniform: null pointer");
+    CA_CHECK(ld_ok(ld, rows), "fill_uniform: ld=%lld < rows=%lld", (long long)ld, (long long)rows);
+    return launch_fill_uniform(x, rows, cols, ld, seed, offset, static_cast<hipStream_t>(stream));
+}
+
 }  // extern "C"

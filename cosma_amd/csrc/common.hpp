@@ -71,6 +71,8 @@ int launch_dgemm_cfg(int cfg, int group_m, char ta, char tb, int64_t m, int64_t 
 void set_dgemm_tuning(int cfg, int group_m);
 int launch_scale(int64_t m, int64_t n, double beta, double *c, int64_t ldc,
                  hipStream_t s);
+int launch_fill_uniform(double *x, int64_t rows, int64_t cols, int64_t ld, uint64_t seed,
+                        uint64_t offset, hipStream_t s);
 int launch_block_transform(char op, int64_t rows, int64_t cols, double alpha,
                            const double *src, int64_t lds, double beta,
                            double *dst, int64_t ldd, hipStream_t s);

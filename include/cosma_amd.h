@@ -105,6 +105,19 @@ int cosma_amd_block_transform(cosma_amd_handle_t handle, char op,
                               const double *src, int64_t ld_src, double beta,
                               double *dst, int64_t ld_dst, void *stream);
 
+/* ------------------------------------------------------- synthetic inputs */
+/*
+ * Fills a column-major rows x cols device matrix (leading dimension ld) with
+ * uniform [-1, 1) values: element (r, c) = u(splitmix64(seed *
+ * 0x9E3779B97F4A7C15 + offset + r + c * rows)), u(x) = ((x >> 11) * 2^-53) * 2 - 1,
+ * exact in FP64 and reproducible on the host.
+ * Replaces the test generator of src/cosma/random_generator.hpp:26-32 (seeded
+ * from std::random_device, not replayable) and the miniapp fill
+ * (miniapp/cosma_miniapp.cpp:21-25).
+ */
+int cosma_amd_fill_uniform(double *x, int64_t rows, int64_t cols, int64_t ld, uint64_t seed,
+                           uint64_t offset, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,75 @@
+"""C-ABI checks that need no GPU: the library loads, exports every entry
+point declared in include/cosma_amd.h, and rejects bad arguments with a
+status code and an error text (the C++ layer turns these into
+std::runtime_error, like the reference's check_runtime_status,
+src/cosma/gpu/utils.hpp:7-13)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+import cosma_amd as ca
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADERS = [os.path.join(ROOT, "include", f) for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h")]
+
+
+def declared_functions():
+    names = set()
+    for h in HEADERS:
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(cosma_amd_\w+)\s*\(", text):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def exported_symbols(path):
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True, check=True).stdout
+    return {line.split()[-1] for line in out.splitlines() if line.strip()}
+
+
+def test_library_loads_and_reports_version():
+    assert os.path.exists(ca.LIB_PATH), "build first: python -m cosma_amd.build"
+    assert ca.lib().cosma_amd_version().decode().startswith("cosma_amd")
+
+
+def test_every_declared_entry_point_is_exported():
+    decl = declared_functions()
+    assert len(decl) >= 6
+    syms = exported_symbols(ca.LIB_PATH)
+    missing = [d for d in decl if d not in syms]
+    assert not missing, f"declared but not exported: {missing}"
+    # and the Python binding knows every one of them
+    assert set(decl) <= set(ca.SIGNATURES), set(decl) - set(ca.SIGNATURES)
+
+
+def test_null_handle_is_rejected_without_touching_the_gpu():
+    L = ca.lib()
+    rc = L.cosma_amd_dgemm(None, b"N", b"N", 4, 4, 4, 1.0, None, 4, None, 4, 0.0, None, 4, None)
+    assert rc == ca.ERR_INVALID_ARG
+    assert "null handle" in L.cosma_amd_last_error().decode()
+    rc = L.cosma_amd_block_transform(None, b"T", 4, 4, 1.0, None, 4, 0.0, None, 4, None)
+    assert rc == ca.ERR_INVALID_ARG
+
+
+def test_bad_arguments_raise_with_text():
+    L = ca.lib()
+    with pytest.raises(ca.CosmaAmdError, match="negative extent"):
+        ca.check(L.cosma_amd_fill_uniform(None, -1, 4, 4, 1, 0, None))
+    with pytest.raises(ca.CosmaAmdError, match="ld="):
+        ca.check(L.cosma_amd_fill_uniform(ctypes.c_void_p(16), 8, 4, 4, 1, 0, None))
+    with pytest.raises(ca.CosmaAmdError, match="tile_config"):
+        ca.set_dgemm_tuning(7, 0)
+    # empty problems are valid no-ops
+    assert L.cosma_amd_fill_uniform(None, 0, 4, 1, 1, 0, None) == ca.OK
+
+
+def test_header_is_plain_c():
+    # The boundary header must compile as C (no C++ or torch types).
+    src = "#include \"cosma_amd.h\"\nint main(void){return 0;}\n"
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), "-x", "c",
+                        "-fsyntax-only", "-"], input=src, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
