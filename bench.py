@@ -1,0 +1,177 @@
+"""Benchmark of the single-rank local multiply on one MI355X.
+
+Workload (the plain-FP64 configuration the headline metric is quoted on):
+C <- alpha * A^T * B + beta * C with m = n = k = 16384, FP64, A, B, C uniform
+[-1, 1) (seeded SplitMix64, generated in HBM), alpha = 1, beta = 0.5,
+operands device-resident.  One step = one DGEMM call through the C ABI.
+
+Prints one JSON line with the throughput (TFLOP/s = 2mnk / t), the measured
+roofline fraction of the DGEMM kernel (FP64 MFMA peak), the error of the
+result against the double-double oracle on sampled rows, and the reference's
+CPU path (one host-BLAS cblas_dgemm, src/cosma/blas.cpp:35) timed on a
+bounded sample on this machine's cores.
+"""
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_MFMA = 78.6  # TFLOP/s, MI355X datasheet dense FP64 matrix
+M = N = K = 16384
+ALPHA, BETA = 1.0, 0.5
+METRIC = "FP64-accurate TFLOP/s, 1 GPU: native DGEMM vs split-precision at stated error"
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    p.add_argument("--cpu-n", type=int, default=8192, help="CPU sample size (m = n = k)")
+    p.add_argument("--no-check", action="store_true", help="skip the sampled-row error check")
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="also run a 2 GiB device copy (block_transform 'N', 2 GiB read + 2 GiB "
+                        "written) to calibrate HBM byte counters")
+    return p.parse_args()
+
+
+def traffic_from_profile():
+    """HBM bytes per launch of the DGEMM kernel from the committed PMC summary, or None."""
+    path = os.path.join(ROOT, "profiles", "dgemm_tn_16384_pmc.md")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"hbm_bytes_per_launch:\s*([0-9.eE+]+)", text)
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(n):
+    """The reference CPU path (one host-BLAS dgemm, op(A) = A^T, beta = 0.5) on n^3."""
+    import numpy as np
+    import oracle
+    threads = min(16, len(os.sched_getaffinity(0)))
+    oracle.hostblas.set_threads(threads)
+    A, B, C = oracle.uniform(n, n, 1), oracle.uniform(n, n, 2), oracle.uniform(n, n, 3)
+    oracle.hostblas.dgemm("T", "N", n, n, n, ALPHA, A, n, B, n, BETA, C, n)   # warm-up
+    times = []
+    budget = time.perf_counter() + 25.0
+    for _ in range(3):
+        t0 = time.perf_counter()
+        oracle.hostblas.dgemm("T", "N", n, n, n, ALPHA, A, n, B, n, BETA, C, n)
+        times.append(time.perf_counter() - t0)
+        if time.perf_counter() > budget:
+            break
+    best = min(times)
+    return {"value": round(2.0 * n ** 3 / best / 1e12, 4), "unit": "TFLOP/s", "cores": threads,
+            "kind": "port",
+            "sample": f"cblas_dgemm('T','N') m=n=k={n}, alpha=1, beta=0.5, uniform[-1,1) inputs, "
+                      f"min of {len(times)} reps after 1 warm-up, BLAS {oracle.hostblas.vendor()}"}
+
+
+def sampled_error(A, B, C, C0_seed, rows=(0, 8191)):
+    """Relative error of sampled rows of C against the double-double oracle."""
+    import numpy as np
+    import oracle
+    Bh = np.asfortranarray(B.cpu().numpy().T)
+    num = den = 0.0
+    for i in rows:
+        a_row = np.asfortranarray(A[i].cpu().numpy().reshape(K, 1))
+        ref = np.zeros((1, N), order="F")
+        oracle.dgemm("T", "N", 1, N, K, ALPHA, a_row, K, Bh, K, 0.0, ref, 1, dd=True)
+        c0 = oracle.uniform_at(C0_seed, i + np.arange(N, dtype=np.uint64) * M)
+        expect = ref[0] + BETA * c0
+        got = C[:, i].cpu().numpy()
+        num += float(((got - expect) ** 2).sum())
+        den += float((expect ** 2).sum())
+    return (num / den) ** 0.5
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import cosma_amd as ca
+
+    dev = torch.device("cuda:0")
+    h = ca.Handle(0)
+    # column-major 16384 x 16384 operands; A is stored k x m (op(A) = A^T)
+    A = torch.empty((M, K), dtype=torch.float64, device=dev)
+    B = torch.empty((N, K), dtype=torch.float64, device=dev)
+    C = torch.empty((N, M), dtype=torch.float64, device=dev)
+    ca.fill_uniform(A.data_ptr(), K, M, K, 11)
+    ca.fill_uniform(B.data_ptr(), K, N, K, 12)
+    ca.fill_uniform(C.data_ptr(), M, N, M, 13)
+    torch.cuda.synchronize()
+
+    if args.calibrate_copy:
+        # 16384 x 16384 doubles = 2 GiB read and 2 GiB written per launch
+        h.block_transform("N", M, N, 1.0, B.data_ptr(), K, 0.0, C.data_ptr(), M)
+        torch.cuda.synchronize()
+        ca.fill_uniform(C.data_ptr(), M, N, M, 13)
+
+    # correctness of one call from the seeded C0 (outside the timed region)
+    h.dgemm("T", "N", M, N, K, ALPHA, A.data_ptr(), K, B.data_ptr(), K, BETA, C.data_ptr(), M)
+    torch.cuda.synchronize()
+    err = None if args.no_check else sampled_error(A, B, C, 13)
+
+    def step():
+        h.dgemm("T", "N", M, N, K, ALPHA, A.data_ptr(), K, B.data_ptr(), K, BETA, C.data_ptr(), M)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    # the kernel is launched on the null stream, which is torch's current stream
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        starts[i].record()
+        step()
+        ends[i].record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = sum(s.elapsed_time(e) for s, e in zip(starts, ends)) / args.steps
+    flops = 2.0 * M * N * K
+    value = flops * args.steps / wall / 1e12
+    achieved = flops / (kernel_ms * 1e-3) / 1e12
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded SplitMix64 uniform[-1,1), generated in HBM)",
+        "config": {"workload": "dgemm op(A)=A^T m=n=k=16384 alpha=1 beta=0.5, device-resident",
+                   "m": M, "n": N, "k": K, "transa": "T", "transb": "N", "parallelism": "single GPU"},
+        "error": {"rel_fro_sampled_rows_vs_double_double": err, "bound_4k_u": 4 * K * 2.0 ** -53},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_MFMA,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_MFMA, 4),
+                     "traffic": traffic_from_profile()},
+    }
+    del A, B, C
+    torch.cuda.empty_cache()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_n)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
