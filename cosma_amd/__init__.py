@@ -44,10 +44,31 @@ SIGNATURES = {
     "cosma_amd_dgemm": (ctypes.c_int, [c_vp, c_char, c_char, c_i64, c_i64, c_i64, c_dbl,
                                        c_vp, c_i64, c_vp, c_i64, c_dbl, c_vp, c_i64, c_vp]),
     "cosma_amd_fill_uniform": (ctypes.c_int, [c_vp, c_i64, c_i64, c_i64, ctypes.c_uint64, ctypes.c_uint64, c_vp]),
+    "cosma_amd_dgemm_host": (ctypes.c_int, [c_vp, c_char, c_char, c_i64, c_i64, c_i64, c_dbl,
+                                            c_vp, c_i64, c_vp, c_i64, c_dbl, c_vp, c_i64, ctypes.c_int,
+                                            ctypes.c_int]),
+    "cosma_amd_get_device_c": (ctypes.c_int, [c_vp, ctypes.POINTER(c_vp), ctypes.POINTER(c_i64)]),
+    "cosma_amd_handle_set_memory_limit": (ctypes.c_int, [c_vp, c_i64]),
+    "cosma_amd_handle_last_mode": (ctypes.c_int, [c_vp]),
     "cosma_amd_dgemm_set_tuning": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "cosma_amd_block_transform": (ctypes.c_int, [c_vp, c_char, c_i64, c_i64, c_dbl, c_vp, c_i64,
                                                  c_dbl, c_vp, c_i64, c_vp]),
 }
+
+
+def _load_torch_runtime_first():
+    """Makes the process use one HIP runtime.
+
+    torch ships its own libamdhip64 with the same SONAME (libamdhip64.so.7) as
+    the system ROCm copy this library links against.  The dynamic loader
+    serves every later request for that SONAME with whichever copy was loaded
+    first, and torch only initialises on its own build, so torch (when it is
+    installed) is imported before the library is loaded.
+    """
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
 
 
 def lib():
@@ -56,6 +77,7 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: run `python -m cosma_amd.build` first")
+        _load_torch_runtime_first()
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)
@@ -98,6 +120,24 @@ class Handle:
     def dgemm(self, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream=None):
         return check(lib().cosma_amd_dgemm(self.ptr, _ch(transa), _ch(transb), m, n, k, alpha,
                                            a, lda, b, ldb, beta, c, ldc, stream))
+
+    def dgemm_host(self, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,
+                   pin_host_buffers=True, copy_c_back=True):
+        """Host-streaming GEMM; a, b, c are raw pointers (host or device)."""
+        return check(lib().cosma_amd_dgemm_host(self.ptr, _ch(transa), _ch(transb), m, n, k, alpha,
+                                                a, lda, b, ldb, beta, c, ldc, int(pin_host_buffers),
+                                                int(copy_c_back)))
+
+    def device_c(self):
+        p, ld = c_vp(), c_i64()
+        check(lib().cosma_amd_get_device_c(self.ptr, ctypes.byref(p), ctypes.byref(ld)))
+        return p.value, ld.value
+
+    def set_memory_limit(self, nbytes):
+        check(lib().cosma_amd_handle_set_memory_limit(self.ptr, nbytes))
+
+    def last_mode(self):
+        return lib().cosma_amd_handle_last_mode(self.ptr)
 
     def block_transform(self, op, rows, cols, alpha, src, ld_src, beta, dst, ld_dst, stream=None):
         return check(lib().cosma_amd_block_transform(self.ptr, _ch(op), rows, cols, alpha, src, ld_src,

@@ -66,15 +66,18 @@ int cosma_amd_handle_create(cosma_amd_handle_t *handle, int device, int streams,
     h->tile_n = tile_n > 0 ? tile_n : 5000;
     h->tile_k = tile_k > 0 ? tile_k : 5000;
     h->n_cu = prop.multiProcessorCount;
-    for (int i = 0; i < h->n_streams; ++i) {
+    for (int i = 0; i < h->n_streams + 2; ++i) {
         hipStream_t s;
         hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
         if (e != hipSuccess) {
             for (auto x : h->streams) hipStreamDestroy(x);
+            if (h->h2d) hipStreamDestroy(h->h2d);
             delete h;
             return hip_fail(e, "hipStreamCreateWithFlags", __FILE__, __LINE__);
         }
-        h->streams.push_back(s);
+        if (i < h->n_streams) h->streams.push_back(s);
+        else if (!h->h2d) h->h2d = s;
+        else h->d2h = s;
     }
     *handle = h;
     return COSMA_AMD_OK;
@@ -88,6 +91,12 @@ int cosma_amd_handle_destroy(cosma_amd_handle_t h) {
         hipStreamSynchronize(s);
         hipStreamDestroy(s);
     }
+    for (auto s : {h->h2d, h->d2h}) {
+        if (!s) continue;
+        hipStreamSynchronize(s);
+        hipStreamDestroy(s);
+    }
+    release_stream_state(h);
     delete h;
     return COSMA_AMD_OK;
 }
@@ -114,6 +123,48 @@ int cosma_amd_dgemm(cosma_amd_handle_t h, char transa, char transb,
     CA_HIP(hipSetDevice(h->device));
     return launch_dgemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, pick_stream(h, stream));
 }
+
+int cosma_amd_dgemm_host(cosma_amd_handle_t h, char transa, char transb,
+                         int64_t m, int64_t n, int64_t k, double alpha,
+                         const double *a, int64_t lda, const double *b, int64_t ldb,
+                         double beta, double *c, int64_t ldc, int pin_host_buffers, int copy_c_back) {
+    (void)pin_host_buffers;   // pageable memory is always staged, never registered in place
+    clear_error();
+    CA_CHECK(h != nullptr, "dgemm_host: null handle");
+    const char ta = norm_trans(transa), tb = norm_trans(transb);
+    CA_CHECK(ta != 0, "dgemm_host: invalid transa '%c'", transa);
+    CA_CHECK(tb != 0, "dgemm_host: invalid transb '%c'", transb);
+    CA_CHECK(m >= 0 && n >= 0 && k >= 0, "dgemm_host: negative extent m=%lld n=%lld k=%lld",
+             (long long)m, (long long)n, (long long)k);
+    if (m == 0 || n == 0) return COSMA_AMD_OK;
+    CA_CHECK(ld_ok(ldc, m), "dgemm_host: ldc=%lld < m=%lld", (long long)ldc, (long long)m);
+    CA_CHECK(c != nullptr, "dgemm_host: null C");
+    if (k > 0 && alpha != 0.0) {
+        CA_CHECK(ld_ok(lda, ta == 'N' ? m : k), "dgemm_host: lda=%lld too small", (long long)lda);
+        CA_CHECK(ld_ok(ldb, tb == 'N' ? k : n), "dgemm_host: ldb=%lld too small", (long long)ldb);
+        CA_CHECK(a != nullptr && b != nullptr, "dgemm_host: null A or B");
+    }
+    CA_HIP(hipSetDevice(h->device));
+    return stream_dgemm(h, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, copy_c_back != 0);
+}
+
+int cosma_amd_get_device_c(cosma_amd_handle_t h, double **c, int64_t *ldc) {
+    clear_error();
+    CA_CHECK(h != nullptr && c != nullptr && ldc != nullptr, "get_device_c: null argument");
+    *c = h->dev_c;
+    *ldc = h->dev_c_ld;
+    return COSMA_AMD_OK;
+}
+
+int cosma_amd_handle_set_memory_limit(cosma_amd_handle_t h, int64_t bytes) {
+    clear_error();
+    CA_CHECK(h != nullptr, "set_memory_limit: null handle");
+    CA_CHECK(bytes >= 0, "set_memory_limit: negative limit");
+    h->mem_limit = bytes;
+    return COSMA_AMD_OK;
+}
+
+int cosma_amd_handle_last_mode(cosma_amd_handle_t h) { return h ? h->last_mode : -1; }
 
 int cosma_amd_dgemm_set_tuning(int tile_config, int group_m) {
     clear_error();

@@ -51,13 +51,42 @@ inline char norm_trans(char t) {
 
 }  // namespace cosma_amd
 
-// Handle: one device, a set of streams, and lazily grown workspaces.
+namespace cosma_amd {
+// A device allocation that grows on demand and is reused across calls.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+// A pinned host staging slot with the event of its last transfer.
+struct StagingSlot {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    hipEvent_t ev = nullptr;
+    // pending copy-out from this slot to pageable host memory (D2H slots)
+    double *out = nullptr;
+    int64_t out_ld = 0, rows = 0, cols = 0;
+};
+}  // namespace cosma_amd
+
+// Handle: one device, its streams and lazily grown workspaces.
 struct cosma_amd_handle_s {
     int device = 0;
     int n_streams = 2;
     int64_t tile_m = 5000, tile_n = 5000, tile_k = 5000;
-    std::vector<hipStream_t> streams;
+    std::vector<hipStream_t> streams;   // compute streams (round-robin)
+    hipStream_t h2d = nullptr;          // host -> device copies
+    hipStream_t d2h = nullptr;          // device -> host copies
     int n_cu = 256;
+    // host-streaming state
+    int64_t mem_limit = 0;              // device bytes the streamer may use (0 = 80% of free)
+    cosma_amd::DevBuf dA, dB, dC;       // resident operand copies / ring slots
+    std::vector<hipEvent_t> events;     // event pool
+    size_t events_used = 0;
+    std::vector<cosma_amd::StagingSlot> h2d_slots, d2h_slots;
+    size_t h2d_next = 0, d2h_next = 0;
+    double *dev_c = nullptr;            // result left on the device (copy_c_back == 0)
+    int64_t dev_c_ld = 0;
+    int last_mode = 0;                  // 0 = device, 1 = resident panels, 2 = bounded ring
 };
 
 namespace cosma_amd {
@@ -73,6 +102,10 @@ int launch_scale(int64_t m, int64_t n, double beta, double *c, int64_t ldc,
                  hipStream_t s);
 int launch_fill_uniform(double *x, int64_t rows, int64_t cols, int64_t ld, uint64_t seed,
                         uint64_t offset, hipStream_t s);
+int stream_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k,
+                 double alpha, const double *a, int64_t lda, const double *b, int64_t ldb,
+                 double beta, double *c, int64_t ldc, bool copy_c_back);
+void release_stream_state(cosma_amd_handle_t h);
 int launch_block_transform(char op, int64_t rows, int64_t cols, double alpha,
                            const double *src, int64_t lds, double beta,
                            double *dst, int64_t ldd, hipStream_t s);

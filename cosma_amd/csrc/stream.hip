@@ -1,0 +1,473 @@
+// Host-tile streaming DGEMM: C <- alpha * op(A) * op(B) + beta * C where any
+// of A, B, C may live in host memory (pinned or pageable) or on the device.
+//
+// Replaces Tiled-MM's gpu::gemm(mm_handle&, ta, tb, m, n, k, alpha, a, lda, b,
+// ldb, beta, c, ldc, pin_host_buffers, copy_c_back), which COSMA calls on
+// host-resident operands (src/cosma/local_multiply.cpp:245-260) after pinning
+// them itself (src/cosma/local_multiply.cpp:341-349).
+//
+// Schedule.  C is cut into tile_m x tile_n tiles and k into tile_k panels
+// (the COSMA_GPU_MAX_TILE_{M,N,K} sizes of the handle).  Three kinds of work
+// run on separate streams and are ordered only by events:
+//   * one host->device stream uploads operand blocks, in the order the tiles
+//     need them;
+//   * compute streams (COSMA_GPU_STREAMS of them, round-robin over C tiles)
+//     run the FP64 MFMA DGEMM of each (tile, panel) as soon as that block's
+//     upload event has fired: panel 0 applies the caller's beta (and reads C
+//     only if beta != 0), later panels accumulate with beta = 1;
+//   * one device->host stream returns each finished C tile while later tiles
+//     are still being uploaded and multiplied (PCIe is full duplex).
+// Resident mode (the default when the operands fit in the device-memory
+// budget — 288 GB of HBM makes this the common case): every block of A and B
+// is uploaded once into a device copy of the whole operand and reused by all
+// tiles that need it, so the host link carries A, B, (C) once instead of
+// re-streaming panels per tile.  Bounded mode (operands larger than the
+// budget): Tiled-MM's scheme — per tile, panels pass through two device
+// slots per operand whose reuse waits on the compute that last read them.
+//
+// Host memory.  Pinned host buffers (hipHostMalloc) are copied directly by DMA.
+// Pageable buffers go through a ring of pinned staging slots: the host packs
+// a block into a slot (several threads for large blocks) and the slot is
+// DMA'd while the next one is packed; returned C tiles are unpacked from
+// their slots once their copy event has fired.  Registering caller memory in
+// place (hipHostRegister, the reference's approach) is not used.
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "common.hpp"
+
+namespace cosma_amd {
+namespace {
+
+enum Kind { DEVICE, PINNED, PAGEABLE };
+
+// COSMA_AMD_TRACE=1 prints the schedule of each streaming call to stderr.
+bool trace_on() {
+    static const bool on = [] {
+        const char *v = std::getenv("COSMA_AMD_TRACE");
+        return v && v[0] == '1';
+    }();
+    return on;
+}
+#define TRACE(...)                                      \
+    do {                                                \
+        if (trace_on()) {                               \
+            std::fprintf(stderr, "[cosma_amd] " __VA_ARGS__); \
+            std::fputc('\n', stderr);                   \
+        }                                               \
+    } while (0)
+
+Kind classify(const void *p) {
+    hipPointerAttribute_t attr;
+    hipError_t e = hipPointerGetAttributes(&attr, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return PAGEABLE;
+    }
+    if (attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged) return DEVICE;
+    if (attr.type == hipMemoryTypeHost) return PINNED;
+    return PAGEABLE;
+}
+
+constexpr size_t SLOT_BYTES = 64ull << 20;   // pinned staging slot
+constexpr int N_SLOTS = 4;                   // per direction
+
+int grow(DevBuf &b, size_t bytes) {
+    if (b.bytes >= bytes) return COSMA_AMD_OK;
+    if (b.ptr) {
+        CA_HIP(hipFree(b.ptr));
+        b.ptr = nullptr;
+        b.bytes = 0;
+    }
+    hipError_t e = hipMalloc(&b.ptr, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("host streaming: hipMalloc of %zu bytes failed (%s)", bytes, hipGetErrorString(e));
+        return COSMA_AMD_ERR_ALLOC;
+    }
+    b.bytes = bytes;
+    return COSMA_AMD_OK;
+}
+
+int get_event(cosma_amd_handle_t h, hipEvent_t *ev) {
+    if (h->events_used == h->events.size()) {
+        hipEvent_t e;
+        CA_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        h->events.push_back(e);
+    }
+    *ev = h->events[h->events_used++];
+    return COSMA_AMD_OK;
+}
+
+int record(cosma_amd_handle_t h, hipStream_t s, hipEvent_t *ev) {
+    int rc = get_event(h, ev);
+    if (rc) return rc;
+    CA_HIP(hipEventRecord(*ev, s));
+    return COSMA_AMD_OK;
+}
+
+int ensure_slots(std::vector<StagingSlot> &slots) {
+    if (!slots.empty()) return COSMA_AMD_OK;
+    for (int i = 0; i < N_SLOTS; ++i) {
+        StagingSlot s;
+        hipError_t e = hipHostMalloc(&s.ptr, SLOT_BYTES, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            set_error("host streaming: pinned staging allocation failed (%s)", hipGetErrorString(e));
+            return COSMA_AMD_ERR_ALLOC;
+        }
+        s.bytes = SLOT_BYTES;
+        CA_HIP(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming));
+        slots.push_back(s);
+    }
+    return COSMA_AMD_OK;
+}
+
+// Column-major 2D copy between host buffers, split over threads by columns.
+void host_copy_2d(double *dst, int64_t dld, const double *src, int64_t sld, int64_t rows, int64_t cols) {
+    const size_t bytes = (size_t)rows * cols * 8;
+    int nt = (int)std::min<size_t>(8, std::max<size_t>(1, bytes >> 22));   // one thread per 4 MB
+    nt = (int)std::min<int64_t>(nt, cols);
+    auto work = [&](int64_t c0, int64_t c1) {
+        for (int64_t c = c0; c < c1; ++c) std::memcpy(dst + c * dld, src + c * sld, rows * 8);
+    };
+    if (nt <= 1) {
+        work(0, cols);
+        return;
+    }
+    std::vector<std::thread> th;
+    const int64_t per = (cols + nt - 1) / nt;
+    for (int t = 0; t < nt; ++t) {
+        const int64_t c0 = t * per, c1 = std::min(cols, c0 + per);
+        if (c0 < c1) th.emplace_back(work, c0, c1);
+    }
+    for (auto &x : th) x.join();
+}
+
+// Finishes a pending copy-out of a D2H staging slot into pageable memory.
+int drain_slot(StagingSlot &s) {
+    if (!s.out) return COSMA_AMD_OK;
+    CA_HIP(hipEventSynchronize(s.ev));
+    host_copy_2d(s.out, s.out_ld, static_cast<const double *>(s.ptr), s.rows, s.rows, s.cols);
+    s.out = nullptr;
+    return COSMA_AMD_OK;
+}
+
+// Chunking of a rows x cols block into pieces that fit one staging slot.
+void chunk_shape(int64_t rows, int64_t cols, int64_t *rc, int64_t *cc) {
+    const int64_t cap = (int64_t)(SLOT_BYTES / 8);
+    *rc = std::min(rows, cap);
+    *cc = std::max<int64_t>(1, std::min(cols, cap / *rc));
+}
+
+// Host block (rows x cols, leading dim sld) -> device (leading dim dld) on h->h2d.
+int upload(cosma_amd_handle_t h, double *dst, int64_t dld, const double *src, int64_t sld,
+           int64_t rows, int64_t cols, Kind kind) {
+    if (rows == 0 || cols == 0) return COSMA_AMD_OK;
+    if (kind == PINNED) {
+        CA_HIP(hipMemcpy2DAsync(dst, dld * 8, src, sld * 8, rows * 8, cols, hipMemcpyHostToDevice, h->h2d));
+        return COSMA_AMD_OK;
+    }
+    int rc = ensure_slots(h->h2d_slots);
+    if (rc) return rc;
+    int64_t rch, cch;
+    chunk_shape(rows, cols, &rch, &cch);
+    for (int64_t c0 = 0; c0 < cols; c0 += cch) {
+        for (int64_t r0 = 0; r0 < rows; r0 += rch) {
+            const int64_t r = std::min(rch, rows - r0), c = std::min(cch, cols - c0);
+            StagingSlot &s = h->h2d_slots[h->h2d_next++ % h->h2d_slots.size()];
+            CA_HIP(hipEventSynchronize(s.ev));   // previous DMA out of this slot is done
+            host_copy_2d(static_cast<double *>(s.ptr), r, src + r0 + c0 * sld, sld, r, c);
+            CA_HIP(hipMemcpy2DAsync(dst + r0 + c0 * dld, dld * 8, s.ptr, r * 8, r * 8, c,
+                                    hipMemcpyHostToDevice, h->h2d));
+            CA_HIP(hipEventRecord(s.ev, h->h2d));
+        }
+    }
+    return COSMA_AMD_OK;
+}
+
+// Device block -> host on h->d2h (pageable destinations finish in drain_slot).
+int download(cosma_amd_handle_t h, double *dst, int64_t dld, const double *src, int64_t sld,
+             int64_t rows, int64_t cols, Kind kind) {
+    if (rows == 0 || cols == 0) return COSMA_AMD_OK;
+    if (kind == PINNED) {
+        CA_HIP(hipMemcpy2DAsync(dst, dld * 8, src, sld * 8, rows * 8, cols, hipMemcpyDeviceToHost, h->d2h));
+        return COSMA_AMD_OK;
+    }
+    int rc = ensure_slots(h->d2h_slots);
+    if (rc) return rc;
+    int64_t rch, cch;
+    chunk_shape(rows, cols, &rch, &cch);
+    for (int64_t c0 = 0; c0 < cols; c0 += cch) {
+        for (int64_t r0 = 0; r0 < rows; r0 += rch) {
+            const int64_t r = std::min(rch, rows - r0), c = std::min(cch, cols - c0);
+            StagingSlot &s = h->d2h_slots[h->d2h_next++ % h->d2h_slots.size()];
+            rc = drain_slot(s);
+            if (rc) return rc;
+            CA_HIP(hipMemcpy2DAsync(s.ptr, r * 8, src + r0 + c0 * sld, sld * 8, r * 8, c,
+                                    hipMemcpyDeviceToHost, h->d2h));
+            CA_HIP(hipEventRecord(s.ev, h->d2h));
+            s.out = dst + r0 + c0 * dld;
+            s.out_ld = dld;
+            s.rows = r;
+            s.cols = c;
+        }
+    }
+    return COSMA_AMD_OK;
+}
+
+int sync_all(cosma_amd_handle_t h) {
+    CA_HIP(hipStreamSynchronize(h->h2d));
+    for (auto s : h->streams) CA_HIP(hipStreamSynchronize(s));
+    CA_HIP(hipStreamSynchronize(h->d2h));
+    for (auto &s : h->d2h_slots) {
+        int rc = drain_slot(s);
+        if (rc) return rc;
+    }
+    return COSMA_AMD_OK;
+}
+
+// Position of block (row0, col0) of op(X) inside the stored (column-major) X.
+inline const double *op_block(const double *x, int64_t ld, char t, int64_t r0, int64_t c0) {
+    return (t == 'N') ? x + r0 + c0 * ld : x + c0 + r0 * ld;
+}
+
+struct Operand {
+    const double *host;   // caller pointer (host or device)
+    int64_t ld;
+    Kind kind;
+    char t;               // 'N' or 'T' (how the stored matrix maps to op(X))
+    int64_t srows, scols; // stored extents
+};
+
+// Uploads block (r0..r0+nr, c0..c0+nc) of op(X) (stored according to X.t) from
+// the host into the device buffer dst (leading dimension dld, same stored
+// orientation), at the block's own position if `at_origin` is false, else at
+// the origin of dst.
+int upload_op_block(cosma_amd_handle_t h, const Operand &X, double *dst, int64_t dld, int64_t r0,
+                    int64_t c0, int64_t nr, int64_t nc, bool at_origin) {
+    const int64_t sr0 = (X.t == 'N') ? r0 : c0, sc0 = (X.t == 'N') ? c0 : r0;
+    const int64_t snr = (X.t == 'N') ? nr : nc, snc = (X.t == 'N') ? nc : nr;
+    double *d = at_origin ? dst : dst + sr0 + sc0 * dld;
+    return upload(h, d, dld, X.host + sr0 + sc0 * X.ld, X.ld, snr, snc, X.kind);
+}
+
+}  // namespace
+
+void release_stream_state(cosma_amd_handle_t h) {
+    hipSetDevice(h->device);
+    for (DevBuf *b : {&h->dA, &h->dB, &h->dC}) {
+        if (b->ptr) hipFree(b->ptr);
+        b->ptr = nullptr;
+        b->bytes = 0;
+    }
+    for (auto *v : {&h->h2d_slots, &h->d2h_slots}) {
+        for (auto &s : *v) {
+            if (s.ev) hipEventDestroy(s.ev);
+            if (s.ptr) hipHostFree(s.ptr);
+        }
+        v->clear();
+    }
+    for (auto e : h->events) hipEventDestroy(e);
+    h->events.clear();
+    h->events_used = 0;
+    h->dev_c = nullptr;
+}
+
+int stream_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k,
+                 double alpha, const double *a, int64_t lda, const double *b, int64_t ldb,
+                 double beta, double *c, int64_t ldc, bool copy_c_back) {
+    if (m == 0 || n == 0) return COSMA_AMD_OK;
+    const bool scale_only = (k == 0 || alpha == 0.0);
+    Operand A{a, lda, scale_only ? DEVICE : classify(a), ta, ta == 'N' ? m : k, ta == 'N' ? k : m};
+    Operand B{b, ldb, scale_only ? DEVICE : classify(b), tb, tb == 'N' ? k : n, tb == 'N' ? n : k};
+    const Kind kc = classify(c);
+    h->events_used = 0;
+    h->dev_c = nullptr;
+
+    if (A.kind == DEVICE && B.kind == DEVICE && kc == DEVICE) {
+        h->last_mode = 0;
+        int rc = launch_dgemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, h->streams[0]);
+        if (rc) return rc;
+        CA_HIP(hipStreamSynchronize(h->streams[0]));
+        return COSMA_AMD_OK;
+    }
+
+    const int64_t tm = std::min(h->tile_m, m), tn = std::min(h->tile_n, n);
+    const int64_t tk = std::max<int64_t>(1, std::min(h->tile_k, k));
+    const int64_t nTm = (m + tm - 1) / tm, nTn = (n + tn - 1) / tn;
+    const int64_t nTk = scale_only ? 0 : (k + tk - 1) / tk;
+    const int S = (int)h->streams.size();
+
+    size_t free_b = 0, total_b = 0;
+    CA_HIP(hipMemGetInfo(&free_b, &total_b));
+    const size_t limit = h->mem_limit > 0 ? (size_t)h->mem_limit : (size_t)(0.8 * (double)free_b);
+    const size_t needA = A.kind == DEVICE ? 0 : (size_t)A.srows * A.scols * 8;
+    const size_t needB = B.kind == DEVICE ? 0 : (size_t)B.srows * B.scols * 8;
+    const size_t needC = kc == DEVICE ? 0 : (size_t)m * n * 8;
+    const bool resident = needA + needB + needC <= limit;
+    h->last_mode = resident ? 1 : 2;
+    TRACE("dgemm_host %c%c m=%lld n=%lld k=%lld alpha=%g beta=%g kinds A=%d B=%d C=%d tiles %lldx%lldx%lld "
+          "grid %lldx%lldx%lld mode=%s", ta, tb, (long long)m, (long long)n, (long long)k, alpha, beta,
+          (int)A.kind, (int)B.kind, (int)kc, (long long)tm, (long long)tn, (long long)tk, (long long)nTm,
+          (long long)nTn, (long long)nTk, resident ? "resident" : "bounded");
+    if (!resident && kc != DEVICE && !copy_c_back) {
+        set_error("host streaming: copy_c_back = false needs C to fit on the device "
+                  "(%zu bytes over a budget of %zu)", needA + needB + needC, limit);
+        return COSMA_AMD_ERR_UNSUPPORTED;
+    }
+    int rc;
+
+    // ---- device storage
+    double *dA = const_cast<double *>(a), *dB = const_cast<double *>(b), *dC = c;
+    int64_t ldA = lda, ldB = ldb, ldC = ldc;
+    if (resident) {
+        if (A.kind != DEVICE) {
+            if ((rc = grow(h->dA, needA))) return rc;
+            dA = static_cast<double *>(h->dA.ptr);
+            ldA = A.srows;
+        }
+        if (B.kind != DEVICE) {
+            if ((rc = grow(h->dB, needB))) return rc;
+            dB = static_cast<double *>(h->dB.ptr);
+            ldB = B.srows;
+        }
+        if (kc != DEVICE) {
+            if ((rc = grow(h->dC, needC))) return rc;
+            dC = static_cast<double *>(h->dC.ptr);
+            ldC = m;
+        }
+    } else {
+        // two slots per streamed operand, one C slot per compute stream
+        const size_t slotA = (size_t)tm * tk * 8, slotB = (size_t)tk * tn * 8, slotC = (size_t)tm * tn * 8;
+        if (A.kind != DEVICE && (rc = grow(h->dA, 2 * slotA))) return rc;
+        if (B.kind != DEVICE && (rc = grow(h->dB, 2 * slotB))) return rc;
+        if (kc != DEVICE && (rc = grow(h->dC, (size_t)S * slotC))) return rc;
+    }
+
+    std::vector<hipEvent_t> evA(resident ? nTm * nTk : 0, nullptr), evB(resident ? nTk * nTn : 0, nullptr);
+    hipEvent_t slot_free_A[2] = {nullptr, nullptr}, slot_free_B[2] = {nullptr, nullptr};
+    std::vector<hipEvent_t> c_slot_free(S, nullptr);
+    int64_t ring = 0;
+
+    for (int64_t i = 0; i < nTm; ++i) {
+        for (int64_t jj = 0; jj < nTn; ++jj) {
+            const int64_t j = (i % 2 == 0) ? jj : nTn - 1 - jj;     // snake: reuse the last B panel
+            const int64_t t = i * nTn + jj;
+            hipStream_t s = h->streams[t % S];
+            const int64_t mi = std::min(tm, m - i * tm), nj = std::min(tn, n - j * tn);
+            const int64_t r0 = i * tm, c0 = j * tn;
+            // C tile location on the device
+            double *Ct;
+            int64_t ldCt;
+            if (kc == DEVICE || resident) {
+                Ct = dC + r0 + c0 * ldC;
+                ldCt = ldC;
+            } else {
+                Ct = static_cast<double *>(h->dC.ptr) + (size_t)(t % S) * tm * tn;
+                ldCt = tm;
+                // the slot's previous tile must have left the device before it is reused
+                if (c_slot_free[t % S]) {
+                    CA_HIP(hipStreamWaitEvent(h->h2d, c_slot_free[t % S], 0));
+                    CA_HIP(hipStreamWaitEvent(s, c_slot_free[t % S], 0));
+                }
+            }
+            TRACE("tile (%lld,%lld) stream %d C at %p ld %lld", (long long)i, (long long)j, (int)(t % S),
+                  (void *)Ct, (long long)ldCt);
+            if (kc != DEVICE && beta != 0.0) {
+                TRACE("  upload C tile %lldx%lld", (long long)mi, (long long)nj);
+                if ((rc = upload(h, Ct, ldCt, c + r0 + c0 * ldc, ldc, mi, nj, kc))) return rc;
+                hipEvent_t ev;
+                if ((rc = record(h, h->h2d, &ev))) return rc;
+                CA_HIP(hipStreamWaitEvent(s, ev, 0));
+            }
+            if (scale_only) {
+                TRACE("  scale by %g", beta);
+                if ((rc = launch_scale(mi, nj, beta, Ct, ldCt, s))) return rc;
+            }
+            for (int64_t p = 0; p < nTk; ++p) {
+                const int64_t k0 = p * tk, kp = std::min(tk, k - k0);
+                const double *Ap, *Bp;
+                int64_t ldAp = ldA, ldBp = ldB;
+                // ---- A block (rows r0..r0+mi of op(A), panel k0..k0+kp)
+                if (A.kind == DEVICE) {
+                    Ap = op_block(a, lda, ta, r0, k0);
+                } else if (resident) {
+                    hipEvent_t &e = evA[i * nTk + p];
+                    if (!e) {
+                        if ((rc = upload_op_block(h, A, dA, ldA, r0, k0, mi, kp, false))) return rc;
+                        if ((rc = record(h, h->h2d, &e))) return rc;
+                    }
+                    CA_HIP(hipStreamWaitEvent(s, e, 0));
+                    Ap = op_block(dA, ldA, ta, r0, k0);
+                } else {
+                    const int q = ring % 2;
+                    double *slot = static_cast<double *>(h->dA.ptr) + (size_t)q * tm * tk;
+                    ldAp = (ta == 'N') ? mi : kp;
+                    if (slot_free_A[q]) CA_HIP(hipStreamWaitEvent(h->h2d, slot_free_A[q], 0));
+                    if ((rc = upload_op_block(h, A, slot, ldAp, r0, k0, mi, kp, true))) return rc;
+                    hipEvent_t e;
+                    if ((rc = record(h, h->h2d, &e))) return rc;
+                    CA_HIP(hipStreamWaitEvent(s, e, 0));
+                    Ap = slot;
+                }
+                // ---- B block (panel k0..k0+kp, columns c0..c0+nj of op(B))
+                if (B.kind == DEVICE) {
+                    Bp = op_block(b, ldb, tb, k0, c0);
+                } else if (resident) {
+                    hipEvent_t &e = evB[p * nTn + j];
+                    if (!e) {
+                        if ((rc = upload_op_block(h, B, dB, ldB, k0, c0, kp, nj, false))) return rc;
+                        if ((rc = record(h, h->h2d, &e))) return rc;
+                    }
+                    CA_HIP(hipStreamWaitEvent(s, e, 0));
+                    Bp = op_block(dB, ldB, tb, k0, c0);
+                } else {
+                    const int q = ring % 2;
+                    double *slot = static_cast<double *>(h->dB.ptr) + (size_t)q * tk * tn;
+                    ldBp = (tb == 'N') ? kp : nj;
+                    if (slot_free_B[q]) CA_HIP(hipStreamWaitEvent(h->h2d, slot_free_B[q], 0));
+                    if ((rc = upload_op_block(h, B, slot, ldBp, k0, c0, kp, nj, true))) return rc;
+                    hipEvent_t e;
+                    if ((rc = record(h, h->h2d, &e))) return rc;
+                    CA_HIP(hipStreamWaitEvent(s, e, 0));
+                    Bp = slot;
+                }
+                rc = launch_dgemm(ta, tb, mi, nj, kp, alpha, Ap, ldAp, Bp, ldBp, p == 0 ? beta : 1.0, Ct, ldCt, s);
+                if (rc) return rc;
+                if (!resident) {
+                    const int q = ring % 2;
+                    hipEvent_t e;
+                    if ((rc = record(h, s, &e))) return rc;
+                    if (A.kind != DEVICE) slot_free_A[q] = e;
+                    if (B.kind != DEVICE) slot_free_B[q] = e;
+                    ++ring;
+                }
+            }
+            if (kc != DEVICE && copy_c_back) {
+                hipEvent_t done;
+                if ((rc = record(h, s, &done))) return rc;
+                CA_HIP(hipStreamWaitEvent(h->d2h, done, 0));
+                TRACE("  download C tile");
+                if ((rc = download(h, c + r0 + c0 * ldc, ldc, Ct, ldCt, mi, nj, kc))) return rc;
+                if (!resident) {
+                    hipEvent_t freed;
+                    if ((rc = record(h, h->d2h, &freed))) return rc;
+                    c_slot_free[t % S] = freed;
+                }
+            }
+        }
+    }
+    if ((rc = sync_all(h))) return rc;
+    if (kc != DEVICE && !copy_c_back) {
+        h->dev_c = dC;
+        h->dev_c_ld = ldC;
+    }
+    return COSMA_AMD_OK;
+}
+
+}  // namespace cosma_amd

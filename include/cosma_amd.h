@@ -77,6 +77,54 @@ int cosma_amd_dgemm(cosma_amd_handle_t handle, char transa, char transb,
                     const double *a, int64_t lda, const double *b, int64_t ldb,
                     double beta, double *c, int64_t ldc, void *stream);
 
+/* ------------------------------------------------- host-streaming GEMM */
+/*
+ * Same contract as cosma_amd_dgemm, but each of A, B, C may be device memory,
+ * pinned host memory (hipHostMalloc) or pageable host memory; the call
+ * returns when C is final (on the host if copy_c_back != 0).
+ * C is cut into tile_m x tile_n tiles and k into tile_k panels (the handle's
+ * tile sizes, i.e. COSMA_GPU_MAX_TILE_{M,N,K}); uploads, per-tile MFMA DGEMMs
+ * on the handle's round-robin streams (COSMA_GPU_STREAMS) and downloads of
+ * finished C tiles overlap.  When A, B and C fit in the device-memory budget
+ * (cosma_amd_handle_set_memory_limit, default 80% of free HBM) every block
+ * crosses the host link once; otherwise blocks are streamed through a
+ * bounded ring of device slots.  Pageable memory is staged through pinned
+ * buffers owned by the handle; pin_host_buffers is accepted for
+ * compatibility and ignored (caller memory is never registered in place).
+ * copy_c_back == 0 leaves C on the device (see cosma_amd_get_device_c).
+ * Replaces Tiled-MM's gpu::gemm(mm_handle<T>&, char ta, char tb, int m, int n,
+ * int k, T alpha, const T* a, int lda, const T* b, int ldb, T beta, T* c,
+ * int ldc, bool pin_host_buffers, bool copy_c_back) as called from
+ * src/cosma/local_multiply.cpp:245-260, with 64-bit extents.
+ */
+int cosma_amd_dgemm_host(cosma_amd_handle_t handle, char transa, char transb,
+                         int64_t m, int64_t n, int64_t k, double alpha,
+                         const double *a, int64_t lda, const double *b, int64_t ldb,
+                         double beta, double *c, int64_t ldc, int pin_host_buffers,
+                         int copy_c_back);
+
+/*
+ * Device copy of C left by the last cosma_amd_dgemm_host call with
+ * copy_c_back == 0 (NULL otherwise), valid until the next streaming call.
+ * Replaces mm_handle<T>::get_full_device_buffer_c() (call site
+ * src/cosma/gpu/nccl_utils.cpp:199-201).
+ */
+int cosma_amd_get_device_c(cosma_amd_handle_t handle, double **c, int64_t *ldc);
+
+/*
+ * Device-memory budget of the host streamer in bytes (0 = 80% of free HBM).
+ * Plays the role of the tile-count bound of Tiled-MM's device buffers
+ * (streams * (tm*tk + tk*tn + tm*tn) elements); no direct counterpart.
+ */
+int cosma_amd_handle_set_memory_limit(cosma_amd_handle_t handle, int64_t bytes);
+
+/*
+ * Schedule used by the last cosma_amd_dgemm_host call: 0 all operands on the
+ * device, 1 resident panels, 2 bounded ring; -1 for a null handle.
+ * Diagnostic; no counterpart in the reference.
+ */
+int cosma_amd_handle_last_mode(cosma_amd_handle_t handle);
+
 /*
  * Selects the DGEMM tile shape for later cosma_amd_dgemm calls (process-wide):
  * tile_config 0 = automatic, 1 = 128x128, 2 = 256x128, 3 = 128x256 (m x n of
