@@ -32,11 +32,13 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", default="device", choices=["device", "host", "tall-skinny"],
+    p.add_argument("--workload", default="device", choices=["device", "host", "tall-skinny", "ozaki"],
                    help="device: 16384^3 TN on device-resident operands (default, the headline); "
                         "host: same GEMM on pinned host operands streamed in 4096^2 tiles; "
                         "tall-skinny: m=n=2048, k=1048576 A^T B streamed from pinned host memory "
-                        "in k-panels of 65536")
+                        "in k-panels of 65536; ozaki: 8192^3 split-precision vs plain FP64 for "
+                        "phi in {0, 1, 4}")
+    p.add_argument("--no-split", action="store_true", help="skip the split-precision summary leg")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     p.add_argument("--cpu-n", type=int, default=8192, help="CPU sample size (m = n = k)")
     p.add_argument("--no-check", action="store_true", help="skip the sampled-row error check")
@@ -176,12 +178,99 @@ def streamed(args):
     print(json.dumps(out), flush=True)
 
 
+def rows_error(Ad, Bd, Cd, n, rows, ref_rows):
+    """Relative Frobenius error of sampled rows of C (device, column-major n x n)."""
+    num = den = 0.0
+    for i, ref in zip(rows, ref_rows):
+        got = Cd[:, i].cpu().numpy()
+        num += float(((got - ref) ** 2).sum())
+        den += float((ref ** 2).sum())
+    return (num / den) ** 0.5
+
+
+def dd_rows(Ad, Bd, n, rows):
+    """Rows of A*B (NN, n x n column-major device operands) from the double-double oracle."""
+    import numpy as np
+    import oracle
+    Bh = np.asfortranarray(Bd.cpu().numpy().T)
+    Ah = Ad.cpu().numpy()                 # tensor (k, m): row l = column l of A
+    out = []
+    for i in rows:
+        a_row = np.asfortranarray(Ah[:, i].reshape(1, n))
+        ref = np.zeros((1, n), order="F")
+        oracle.dgemm("N", "N", 1, n, n, 1.0, a_row, 1, Bh, n, 0.0, ref, 1, dd=True)
+        out.append(ref[0])
+    return out
+
+
+def split_precision(phis=(0.0, 1.0, 4.0), steps=5, warmup=1, n=8192, rows=(0, 4097)):
+    """8192^3 C = A*B (NN): plain FP64 MFMA DGEMM vs the split-precision DGEMM
+    (moduli at the default 1e-14 tolerance; slices with 7 slices), inputs
+    a = (u - 0.5) exp(phi z).  Errors are relative Frobenius errors of sampled
+    rows against the double-double oracle."""
+    import torch
+    import cosma_amd as ca
+    h = ca.Handle(0)
+    A = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    B = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    C = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    flops = 2.0 * n ** 3
+    results = []
+    for phi in phis:
+        ca.fill_lognormal(A.data_ptr(), n, n, n, 21, phi)
+        ca.fill_lognormal(B.data_ptr(), n, n, n, 22, phi)
+        torch.cuda.synchronize()
+        ref = dd_rows(A, B, n, rows)
+
+        def timed(fn):
+            for _ in range(warmup):
+                fn()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                fn()
+            torch.cuda.synchronize()
+            return (time.perf_counter() - t0) / steps
+
+        plain = lambda: h.dgemm("N", "N", n, n, n, 1.0, A.data_ptr(), n, B.data_ptr(), n, 0.0, C.data_ptr(), n)
+        t_plain = timed(plain)
+        e_plain = rows_error(A, B, C, n, rows, ref)
+        entry = {"phi": phi, "fp64": {"tflops": round(flops / t_plain / 1e12, 3), "ms": round(t_plain * 1e3, 3),
+                                      "rel_err": e_plain}}
+        for name, method, count in (("moduli", ca.OZAKI_MODULI, 0), ("slices", ca.OZAKI_SLICES, 7)):
+            info = {}
+
+            def oz():
+                info.update(h.dgemm_ozaki("N", "N", n, n, n, 1.0, A.data_ptr(), n, B.data_ptr(), n, 0.0,
+                                          C.data_ptr(), n, method=method, tolerance=1e-14, count=count))
+            t = timed(oz)
+            entry[name] = {"tflops_fp64_equiv": round(flops / t / 1e12, 3), "ms": round(t * 1e3, 3),
+                           "speedup_vs_fp64": round(t_plain / t, 3), "rel_err": rows_error(A, B, C, n, rows, ref),
+                           "count": info["count"], "bits": info["bits"], "int8_products": info["int8_products"],
+                           "error_estimate": info["error_bound"]}
+        results.append(entry)
+    del A, B, C
+    torch.cuda.empty_cache()
+    return results
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "ozaki":
+        res = split_precision(steps=args.steps, warmup=args.warmup)
+        best = max(r["moduli"]["tflops_fp64_equiv"] for r in res)
+        out = {"metric": METRIC, "value": best, "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps,
+               "warmup": args.warmup, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+               "dtype": "f64 (int8 MFMA inside)", "data": "synthetic a=(u-0.5)exp(phi z), seeded",
+               "config": {"workload": "C=A*B m=n=k=8192, plain FP64 vs split precision, phi in {0,1,4}"},
+               "results": res,
+               "error_note": "rel_err = relative Frobenius error of 2 sampled rows vs the double-double oracle"}
+        print(json.dumps(out), flush=True)
+        return
     if args.workload != "device":
         return streamed(args)
     import torch
@@ -253,6 +342,9 @@ def main():
     }
     del A, B, C
     torch.cuda.empty_cache()
+    if not args.no_split:
+        sp = split_precision(phis=(0.0,), steps=3, warmup=1)[0]
+        out["split_precision"] = {"workload": "C=A*B m=n=k=8192 NN, uniform inputs (phi=0)", **sp}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n)
     print(json.dumps(out), flush=True)

@@ -35,6 +35,18 @@ class CosmaAmdError(RuntimeError):
         self.status = status
 
 
+OZAKI_MODULI = 0
+OZAKI_SLICES = 1
+
+
+class OzakiInfo(ctypes.Structure):
+    _fields_ = [("method", ctypes.c_int), ("count", ctypes.c_int), ("bits", ctypes.c_int),
+                ("int8_products", ctypes.c_int64), ("error_bound", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "cosma_amd_last_error": (ctypes.c_char_p, []),
@@ -44,12 +56,16 @@ SIGNATURES = {
     "cosma_amd_dgemm": (ctypes.c_int, [c_vp, c_char, c_char, c_i64, c_i64, c_i64, c_dbl,
                                        c_vp, c_i64, c_vp, c_i64, c_dbl, c_vp, c_i64, c_vp]),
     "cosma_amd_fill_uniform": (ctypes.c_int, [c_vp, c_i64, c_i64, c_i64, ctypes.c_uint64, ctypes.c_uint64, c_vp]),
+    "cosma_amd_dgemm_ozaki": (ctypes.c_int, [c_vp, c_char, c_char, c_i64, c_i64, c_i64, c_dbl,
+                                             c_vp, c_i64, c_vp, c_i64, c_dbl, c_vp, c_i64, ctypes.c_int,
+                                             c_dbl, ctypes.c_int, ctypes.c_void_p, c_vp]),
     "cosma_amd_dgemm_host": (ctypes.c_int, [c_vp, c_char, c_char, c_i64, c_i64, c_i64, c_dbl,
                                             c_vp, c_i64, c_vp, c_i64, c_dbl, c_vp, c_i64, ctypes.c_int,
                                             ctypes.c_int]),
     "cosma_amd_get_device_c": (ctypes.c_int, [c_vp, ctypes.POINTER(c_vp), ctypes.POINTER(c_i64)]),
     "cosma_amd_handle_set_memory_limit": (ctypes.c_int, [c_vp, c_i64]),
     "cosma_amd_handle_last_mode": (ctypes.c_int, [c_vp]),
+    "cosma_amd_fill_lognormal": (ctypes.c_int, [c_vp, c_i64, c_i64, c_i64, ctypes.c_uint64, c_dbl, c_vp]),
     "cosma_amd_dgemm_set_tuning": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "cosma_amd_block_transform": (ctypes.c_int, [c_vp, c_char, c_i64, c_i64, c_dbl, c_vp, c_i64,
                                                  c_dbl, c_vp, c_i64, c_vp]),
@@ -121,6 +137,15 @@ class Handle:
         return check(lib().cosma_amd_dgemm(self.ptr, _ch(transa), _ch(transb), m, n, k, alpha,
                                            a, lda, b, ldb, beta, c, ldc, stream))
 
+    def dgemm_ozaki(self, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,
+                    method=OZAKI_MODULI, tolerance=1e-14, count=0, stream=None):
+        """Split-precision GEMM on device pointers; returns the OzakiInfo as a dict."""
+        info = OzakiInfo()
+        check(lib().cosma_amd_dgemm_ozaki(self.ptr, _ch(transa), _ch(transb), m, n, k, alpha, a, lda, b, ldb,
+                                          beta, c, ldc, method, tolerance, count, ctypes.addressof(info),
+                                          stream))
+        return info.as_dict()
+
     def dgemm_host(self, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,
                    pin_host_buffers=True, copy_c_back=True):
         """Host-streaming GEMM; a, b, c are raw pointers (host or device)."""
@@ -147,6 +172,11 @@ class Handle:
 def fill_uniform(ptr, rows, cols, ld, seed, offset=0, stream=None):
     """Seeded uniform [-1, 1) fill of a column-major device matrix."""
     check(lib().cosma_amd_fill_uniform(ptr, rows, cols, ld, seed, offset, stream))
+
+
+def fill_lognormal(ptr, rows, cols, ld, seed, phi, stream=None):
+    """a = (u - 0.5) * exp(phi * z) fill of a column-major device matrix."""
+    check(lib().cosma_amd_fill_lognormal(ptr, rows, cols, ld, seed, phi, stream))
 
 
 def set_dgemm_tuning(tile_config=0, group_m=0):

@@ -148,6 +148,32 @@ int cosma_amd_dgemm_host(cosma_amd_handle_t h, char transa, char transb,
     return stream_dgemm(h, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, copy_c_back != 0);
 }
 
+int cosma_amd_dgemm_ozaki(cosma_amd_handle_t h, char transa, char transb,
+                          int64_t m, int64_t n, int64_t k, double alpha,
+                          const double *a, int64_t lda, const double *b, int64_t ldb,
+                          double beta, double *c, int64_t ldc, int method, double tolerance,
+                          int count, cosma_amd_ozaki_info *info, void *stream) {
+    clear_error();
+    CA_CHECK(h != nullptr, "dgemm_ozaki: null handle");
+    const char ta = norm_trans(transa), tb = norm_trans(transb);
+    CA_CHECK(ta != 0, "dgemm_ozaki: invalid transa '%c'", transa);
+    CA_CHECK(tb != 0, "dgemm_ozaki: invalid transb '%c'", transb);
+    CA_CHECK(method == COSMA_AMD_OZAKI_MODULI || method == COSMA_AMD_OZAKI_SLICES,
+             "dgemm_ozaki: unknown method %d", method);
+    CA_CHECK(m >= 0 && n >= 0 && k >= 0, "dgemm_ozaki: negative extent");
+    if (m == 0 || n == 0) return COSMA_AMD_OK;
+    CA_CHECK(ld_ok(ldc, m), "dgemm_ozaki: ldc=%lld < m=%lld", (long long)ldc, (long long)m);
+    CA_CHECK(c != nullptr, "dgemm_ozaki: null C");
+    if (k > 0 && alpha != 0.0) {
+        CA_CHECK(ld_ok(lda, ta == 'N' ? m : k), "dgemm_ozaki: lda=%lld too small", (long long)lda);
+        CA_CHECK(ld_ok(ldb, tb == 'N' ? k : n), "dgemm_ozaki: ldb=%lld too small", (long long)ldb);
+        CA_CHECK(a != nullptr && b != nullptr, "dgemm_ozaki: null A or B");
+    }
+    CA_HIP(hipSetDevice(h->device));
+    return ozaki_dgemm(h, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, method, tolerance, count,
+                       info, pick_stream(h, stream));
+}
+
 int cosma_amd_get_device_c(cosma_amd_handle_t h, double **c, int64_t *ldc) {
     clear_error();
     CA_CHECK(h != nullptr && c != nullptr && ldc != nullptr, "get_device_c: null argument");
@@ -198,6 +224,16 @@ int cosma_amd_fill_uniform(double *x, int64_t rows, int64_t cols, int64_t ld, ui
     CA_CHECK(x != nullptr, "fill_uniform: null pointer");
     CA_CHECK(ld_ok(ld, rows), "fill_uniform: ld=%lld < rows=%lld", (long long)ld, (long long)rows);
     return launch_fill_uniform(x, rows, cols, ld, seed, offset, static_cast<hipStream_t>(stream));
+}
+
+int cosma_amd_fill_lognormal(double *x, int64_t rows, int64_t cols, int64_t ld, uint64_t seed, double phi,
+                             void *stream) {
+    clear_error();
+    CA_CHECK(rows >= 0 && cols >= 0, "fill_lognormal: negative extent");
+    if (rows == 0 || cols == 0) return COSMA_AMD_OK;
+    CA_CHECK(x != nullptr, "fill_lognormal: null pointer");
+    CA_CHECK(ld_ok(ld, rows), "fill_lognormal: ld=%lld < rows=%lld", (long long)ld, (long long)rows);
+    return launch_fill_lognormal(x, rows, cols, ld, seed, phi, static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

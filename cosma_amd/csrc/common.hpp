@@ -80,6 +80,7 @@ struct cosma_amd_handle_s {
     // host-streaming state
     int64_t mem_limit = 0;              // device bytes the streamer may use (0 = 80% of free)
     cosma_amd::DevBuf dA, dB, dC;       // resident operand copies / ring slots
+    cosma_amd::DevBuf ozaki_ws;         // split-precision workspace
     std::vector<hipEvent_t> events;     // event pool
     size_t events_used = 0;
     std::vector<cosma_amd::StagingSlot> h2d_slots, d2h_slots;
@@ -106,6 +107,12 @@ int stream_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, i
                  double alpha, const double *a, int64_t lda, const double *b, int64_t ldb,
                  double beta, double *c, int64_t ldc, bool copy_c_back);
 void release_stream_state(cosma_amd_handle_t h);
+int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha,
+                const double *a, int64_t lda, const double *b, int64_t ldb, double beta, double *c,
+                int64_t ldc, int method, double tol, int count, cosma_amd_ozaki_info *info, hipStream_t s);
+int moduli_bits(int n, int64_t k);
+int launch_fill_lognormal(double *x, int64_t rows, int64_t cols, int64_t ld, uint64_t seed, double phi,
+                          hipStream_t s);
 int launch_block_transform(char op, int64_t rows, int64_t cols, double alpha,
                            const double *src, int64_t lds, double beta,
                            double *dst, int64_t ldd, hipStream_t s);

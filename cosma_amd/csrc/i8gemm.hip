@@ -1,0 +1,233 @@
+// Batched INT8 MFMA GEMM with pair lists and residue / FP64 epilogues.
+// See i8gemm.hpp for the contract and the tile layout.
+#include "i8gemm.hpp"
+
+namespace cosma_amd {
+namespace {
+
+constexpr int BM = 256, BN = 256, BK = 64, NT = 256;
+constexpr int NXCD = 8;
+constexpr int GROUP_M = 4;
+
+__device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t T) {
+    const int64_t xcd = b % NXCD, local = b / NXCD;
+    const int64_t q = T / NXCD, r = T % NXCD;
+    return (xcd < r) ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
+}
+
+// Symmetric residue of x modulo m, in [-(m/2), (m-1)/2].  The quotient comes
+// from an FP32 estimate: for |x| < 2^31 and m >= 2 the estimate is within one
+// of the rounded quotient, so one correction on each side is exact.
+__device__ __forceinline__ int sym_mod(int x, int m, float inv_m) {
+    const int q = (int)rintf((float)x * inv_m);
+    int r = x - q * m;
+    r -= (r > (m - 1) / 2) ? m : 0;
+    r += (r < -(m / 2)) ? m : 0;
+    return r;
+}
+
+template <int EPI>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void i8gemm_kernel(I8Problem p, I8Jobs jobs) {
+    // LDS tiles: rows of 64 bytes (one k-step), unpadded, with the four
+    // 16-byte chunks of row r stored in the order chunk ^ ((r >> 1) & 3).
+    // Stores (four consecutive lanes per row, matching 64-byte global runs)
+    // fill a contiguous 1 KB per wave instruction, and the MFMA fragment read
+    // (lane 16g + i: row i, chunk g) is free of bank conflicts on gfx950
+    // (measured with SQ_LDS_BANK_CONFLICT; padded rows cost ~1.3 conflict
+    // cycles per read and 2.6 per store).
+    __shared__ __attribute__((aligned(16))) i32x4 sa[2][BM * BK / 16];
+    __shared__ __attribute__((aligned(16))) i32x4 sb[2][BN * BK / 16];
+
+    const int z = blockIdx.y;
+    const int64_t tiles_m = (p.M + BM - 1) / BM, tiles_n = (p.N + BN - 1) / BN;
+    const int64_t bid = xcd_remap(blockIdx.x, tiles_m * tiles_n);
+    const int64_t per_group = (int64_t)GROUP_M * tiles_n;
+    const int64_t grp = bid / per_group;
+    const int64_t first_m = grp * GROUP_M;
+    const int64_t gsize = min((int64_t)GROUP_M, tiles_m - first_m);
+    const int64_t in_group = bid - grp * per_group;
+    const int64_t m0 = (first_m + in_group % gsize) * BM;
+    const int64_t n0 = (in_group / gsize) * BN;
+
+    const int t = threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int g = lane >> 4, li = lane & 15;
+
+    // accumulators zeroed by an MFMA so they live in AGPRs across the loop
+    i32x4 acc[8][8];
+    const i32x4 zero4 = {0, 0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(zero4, zero4, zero4, 0, 0, 0);
+
+    const int ksteps = (int)(p.K / BK);
+    const int nsteps = jobs.npairs[z] * ksteps;
+    // slice indices of the pairs, held in registers (byte p of the word = pair p)
+    uint64_t pa_bits = 0, pb_bits = 0;
+#pragma unroll
+    for (int q = 0; q < I8_MAX_PAIRS; ++q) {
+        pa_bits |= (uint64_t)jobs.pa[z][q] << (8 * q);
+        pb_bits |= (uint64_t)jobs.pb[z][q] << (8 * q);
+    }
+
+    // global -> register staging: 4 x 16 bytes of A and of B per thread and
+    // step, two sets so that a step's loads are issued two steps before they
+    // are written to LDS (step t uses set t & 1)
+    i32x4 ra[2][4], rb[2][4];
+    auto load_step = [&](int step, int set) {
+        const int pair = step / ksteps;
+        const int64_t k0 = (int64_t)(step - pair * ksteps) * BK;
+        const int8_t *A = p.a + (int64_t)((pa_bits >> (8 * pair)) & 0xff) * p.a_stride;
+        const int8_t *B = p.b + (int64_t)((pb_bits >> (8 * pair)) & 0xff) * p.b_stride;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int idx = t + NT * q;
+            const int row = idx >> 2, chunk = idx & 3;
+            ra[set][q] = *reinterpret_cast<const i32x4 *>(A + (m0 + row) * p.lda + k0 + chunk * 16);
+            rb[set][q] = *reinterpret_cast<const i32x4 *>(B + (n0 + row) * p.ldb + k0 + chunk * 16);
+        }
+    };
+    auto swz = [](int row, int chunk) { return row * 4 + (chunk ^ ((row >> 1) & 3)); };
+    auto store_step = [&](int st, int set) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int idx = t + NT * q;
+            const int row = idx >> 2, chunk = idx & 3;
+            sa[st][swz(row, chunk)] = ra[set][q];
+            sb[st][swz(row, chunk)] = rb[set][q];
+        }
+    };
+
+    // Fragment registers, double-buffered across k-steps: while the MFMAs of
+    // step s run on F[s & 1], the fragments of step s + 1 are read from the
+    // other LDS stage into F[(s + 1) & 1].
+    i32x4 fa[2][8], fb[2][8];
+    auto read_frags = [&](int st) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) fb[st][j] = sb[st][swz(wn * 128 + 16 * j + li, g)];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) fa[st][i] = sa[st][swz(wm * 128 + 16 * i + li, g)];
+    };
+    auto mfmas = [&](int st) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[st][i], fb[st][j], acc[i][j], 0, 0, 0);
+    };
+    // Step s lives in LDS stage s & 1.  Iteration s: read the fragments of step
+    // s + 1, run the MFMAs of step s, write step s + 2 (register set s & 1)
+    // into stage s & 1 (its fragments were read before the previous barrier),
+    // load step s + 4 into the register set just freed, barrier.
+    // The body is branch-free (loads past the end re-read the last step, and
+    // the extra stores and fragment reads are never consumed) so that the
+    // compiler's wait counts only cover the set being stored: a guarded body
+    // made it wait for the newer prefetch set as well, halving the prefetch
+    // distance.  nsteps is even (K is a multiple of 128).
+    const int last = nsteps - 1;
+    auto iteration = [&](int s, int st) {
+        read_frags(st ^ 1);
+        mfmas(st);
+        store_step(st, st);
+        load_step(min(s + 4, last), st);
+        __syncthreads();
+    };
+
+    if (nsteps > 0) {
+        load_step(0, 0);
+        load_step(min(1, last), 1);
+        store_step(0, 0);
+        load_step(min(2, last), 0);
+        __syncthreads();
+        read_frags(0);
+        store_step(1, 1);
+        load_step(min(3, last), 1);
+        __syncthreads();
+        for (int s = 0; s < nsteps; s += 2) {
+            iteration(s, 0);
+            iteration(s + 1, 1);
+        }
+    }
+
+    // ---- epilogue: lane holds rows 4g..4g+3 (contiguous) of column li
+    const int mod = jobs.modulus[z];
+    const float inv_m = 1.0f / (float)(mod > 0 ? mod : 1);
+    const double scale = jobs.scale[z];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int64_t row = m0 + wm * 128 + 16 * i + 4 * g;
+        if (row >= p.M) continue;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int64_t col = n0 + wn * 128 + 16 * j + li;
+            if (col >= p.N) continue;
+            const i32x4 r = acc[i][j];
+            const int64_t off = (int64_t)z * p.out_stride + row + col * p.ldo;
+            const bool full = row + 3 < p.M;
+            if (EPI == EPI_I32) {
+                int *o = static_cast<int *>(p.out) + off;
+                if (full) *reinterpret_cast<i32x4 *>(o) = r;
+                else for (int e = 0; e < 4 && row + e < p.M; ++e) o[e] = r[e];
+            } else if (EPI == EPI_RES8) {
+                int8_t *o = static_cast<int8_t *>(p.out) + off;
+                int8_t v[4];
+                for (int e = 0; e < 4; ++e) v[e] = (int8_t)sym_mod(r[e], mod, inv_m);
+                if (full) {
+                    *reinterpret_cast<uint32_t *>(o) = (uint32_t)(uint8_t)v[0] | ((uint32_t)(uint8_t)v[1] << 8) |
+                                                      ((uint32_t)(uint8_t)v[2] << 16) | ((uint32_t)(uint8_t)v[3] << 24);
+                } else {
+                    for (int e = 0; e < 4 && row + e < p.M; ++e) o[e] = v[e];
+                }
+            } else {
+                double *o = static_cast<double *>(p.out) + off;
+                for (int e = 0; e < 4 && row + e < p.M; ++e) {
+                    const double val = scale * (double)r[e];
+                    o[e] = p.accumulate ? o[e] + val : val;
+                }
+            }
+        }
+    }
+}
+
+}  // namespace
+
+int launch_i8gemm(const I8Problem &p, const I8Jobs &jobs, int Z, I8Epilogue epi, hipStream_t s) {
+    if (p.M == 0 || p.N == 0 || Z == 0) return COSMA_AMD_OK;
+    if (Z > I8_MAX_JOBS) {
+        set_error("i8gemm: %d jobs exceed the limit of %d", Z, I8_MAX_JOBS);
+        return COSMA_AMD_ERR_UNSUPPORTED;
+    }
+    auto a16 = [](const void *x) { return (reinterpret_cast<uintptr_t>(x) & 15) == 0; };
+    if (p.K % (2 * BK) || p.lda % 16 || p.ldb % 16 || p.a_stride % 16 || p.b_stride % 16 || !a16(p.a) || !a16(p.b)) {
+        set_error("i8gemm: K must be a multiple of 128 and strides/bases 16-byte aligned");
+        return COSMA_AMD_ERR_INVALID_ARG;
+    }
+    if ((epi == EPI_I32 || epi == EPI_RES8) && (p.ldo % 4 || p.out_stride % 4 ||
+                                                 (reinterpret_cast<uintptr_t>(p.out) & 3))) {
+        set_error("i8gemm: output leading dimension must be a multiple of 4");
+        return COSMA_AMD_ERR_INVALID_ARG;
+    }
+    for (int z = 0; z < Z; ++z) {
+        if (jobs.npairs[z] < 0 || jobs.npairs[z] > I8_MAX_PAIRS) {
+            set_error("i8gemm: job %d has %d pairs", z, jobs.npairs[z]);
+            return COSMA_AMD_ERR_INVALID_ARG;
+        }
+        if (epi == EPI_RES8 && (jobs.modulus[z] < 2 || jobs.modulus[z] > 256)) {
+            set_error("i8gemm: modulus %d out of range", jobs.modulus[z]);
+            return COSMA_AMD_ERR_INVALID_ARG;
+        }
+    }
+    const int64_t tiles = ((p.M + BM - 1) / BM) * ((p.N + BN - 1) / BN);
+    const dim3 grid((unsigned)tiles, (unsigned)Z), block(NT);
+    switch (epi) {
+    case EPI_I32: hipLaunchKernelGGL(i8gemm_kernel<EPI_I32>, grid, block, 0, s, p, jobs); break;
+    case EPI_RES8: hipLaunchKernelGGL(i8gemm_kernel<EPI_RES8>, grid, block, 0, s, p, jobs); break;
+    default: hipLaunchKernelGGL(i8gemm_kernel<EPI_F64ACC>, grid, block, 0, s, p, jobs); break;
+    }
+    CA_HIP(hipGetLastError());
+    return COSMA_AMD_OK;
+}
+
+}  // namespace cosma_amd

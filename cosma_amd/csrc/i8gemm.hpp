@@ -1,0 +1,67 @@
+// Batched INT8 x INT8 -> INT32 GEMM on the gfx950 matrix cores
+// (v_mfma_i32_16x16x64_i8), the engine of the split-precision DGEMM.
+//
+// Operands are k-contiguous int8 matrices: A is M x K stored as rows of K
+// bytes (row i at A + i * lda), B is N x K stored as columns of K bytes
+// (column j at B + j * ldb).  All slices (or residues) of one operand live in
+// one allocation, slice s at base + s * slice_stride.  The split kernels write
+// them in this layout, so every global load is a 16-byte run along k.
+//
+// One launch computes, for each job z in [0, Z), an M x N result
+//     acc_z = sum over the pairs p of job z of  A[pa[z][p]] * B[pb[z][p]]^T
+// (a pair list concatenates products along k without moving data), and hands
+// acc_z to an epilogue:
+//   EPI_I32     store the INT32 result (tests);
+//   EPI_RES8    reduce modulo the job's modulus into the symmetric range and
+//               store one int8 residue per element (moduli method);
+//   EPI_F64ACC  C64 = scale * acc (+ C64 if accumulate) in FP64 (slices method;
+//               jobs of one launch must write disjoint outputs).
+//
+// Tile: 256 x 256 per workgroup of 256 threads (2 x 2 waves of 128 x 128, i.e.
+// 8 x 8 MFMA blocks of 16 x 16 per wave, 256 INT32 accumulators per lane in
+// AGPRs), k-step 64 (one MFMA depth).  LDS tiles use unpadded 64-byte rows
+// with a chunk swizzle (chunk ^ ((row >> 1) & 3)) that keeps the fragment
+// reads and the stores free of bank conflicts.
+// MFMA i32 16x16x64 lane layout (checked on gfx950): lane l = 16g + i supplies
+// A row i / B column i, bytes k = 16g .. 16g+15; it receives D rows 4g..4g+3 of
+// column i.
+#pragma once
+#include "common.hpp"
+
+namespace cosma_amd {
+
+constexpr int I8_MAX_PAIRS = 8;
+constexpr int I8_MAX_JOBS = 24;
+
+enum I8Epilogue { EPI_I32 = 0, EPI_RES8 = 1, EPI_F64ACC = 2 };
+
+struct I8Jobs {
+    int npairs[I8_MAX_JOBS];
+    unsigned char pa[I8_MAX_JOBS][I8_MAX_PAIRS];
+    unsigned char pb[I8_MAX_JOBS][I8_MAX_PAIRS];
+    int modulus[I8_MAX_JOBS];
+    double scale[I8_MAX_JOBS];
+};
+
+struct I8Problem {
+    int64_t M, N, K;
+    const int8_t *a;      // A slice 0
+    int64_t lda;          // bytes between rows of A
+    int64_t a_stride;     // bytes between slices of A
+    const int8_t *b;
+    int64_t ldb;
+    int64_t b_stride;
+    void *out;            // output of job 0 (int32 / int8 / double, column-major)
+    int64_t ldo;          // leading dimension of the outputs (elements)
+    int64_t out_stride;   // elements between the outputs of consecutive jobs
+    int accumulate;       // EPI_F64ACC: add into the output
+};
+
+// Requirements: K % 128 == 0; lda, ldb, a_stride, b_stride multiples of 16 and
+// 16-byte aligned bases; int8/int32 outputs with ldo and out_stride multiples
+// of 4 (all checked).  Not checkable, hence the callers' duty: every A slice
+// holds ceil(M/256)*256 rows and every B slice ceil(N/256)*256 columns (the
+// split kernels pad with zeros), so tile loads never need bounds checks.
+int launch_i8gemm(const I8Problem &p, const I8Jobs &jobs, int Z, I8Epilogue epi, hipStream_t s);
+
+}  // namespace cosma_amd

@@ -77,6 +77,46 @@ int cosma_amd_dgemm(cosma_amd_handle_t handle, char transa, char transb,
                     const double *a, int64_t lda, const double *b, int64_t ldb,
                     double beta, double *c, int64_t ldc, void *stream);
 
+/* ------------------------------------------------ split-precision GEMM */
+enum {
+    COSMA_AMD_OZAKI_MODULI = 0,   /* integer residues modulo coprime moduli <= 256, CRT recombination */
+    COSMA_AMD_OZAKI_SLICES = 1    /* signed 7-bit slices, triangular slice products */
+};
+
+typedef struct {
+    int method;             /* method used; -1 if A or B held Inf/NaN (FP64 MFMA DGEMM used instead) */
+    int count;              /* moduli or slices used */
+    int bits;               /* bits kept per input entry relative to its row/column maximum */
+    int64_t int8_products;  /* INT8 GEMMs of size m x n x k computed */
+    double error_bound;     /* a-priori relative error estimate for inputs comparable to their maxima */
+} cosma_amd_ozaki_info;
+
+/*
+ * FP64-accurate C <- alpha * op(A) * op(B) + beta * C on device-resident
+ * operands, computed on the INT8 matrix cores (v_mfma_i32_16x16x64_i8) with
+ * exact INT32 accumulation.  Rows of op(A) and columns of op(B) are scaled by
+ * powers of two (exponent of their maximum), the scaled entries are split into
+ * INT8 operands (residues or slices, see the enum), the INT8 products are
+ * computed, and FP64 recombines them and applies alpha and beta (beta == 0
+ * never reads C).  FP64 arithmetic runs only in the scaling, the
+ * recombination and the epilogue.
+ * tolerance: requested relative accuracy (<= 0 means 1e-14); count > 0 forces
+ * the number of moduli (2..16) or slices (1..8) instead.  info (may be NULL)
+ * receives the method, count, bits and the error estimate.
+ * Limits: moduli need k < 131072 and slices need slices * k * 127^2 < 2^31
+ * (exact INT32 sums); COSMA_AMD_ERR_UNSUPPORTED otherwise.
+ * Uses a workspace owned by the handle; asynchronous except for one 4-byte
+ * read-back that detects non-finite inputs.
+ * No counterpart in the reference (its local multiply is a host or vendor
+ * FP64 GEMM, src/cosma/local_multiply.cpp:299-381); same argument
+ * conventions as cosma_amd_dgemm.
+ */
+int cosma_amd_dgemm_ozaki(cosma_amd_handle_t handle, char transa, char transb,
+                          int64_t m, int64_t n, int64_t k, double alpha,
+                          const double *a, int64_t lda, const double *b, int64_t ldb,
+                          double beta, double *c, int64_t ldc, int method, double tolerance,
+                          int count, cosma_amd_ozaki_info *info, void *stream);
+
 /* ------------------------------------------------- host-streaming GEMM */
 /*
  * Same contract as cosma_amd_dgemm, but each of A, B, C may be device memory,
@@ -165,6 +205,17 @@ int cosma_amd_block_transform(cosma_amd_handle_t handle, char op,
  */
 int cosma_amd_fill_uniform(double *x, int64_t rows, int64_t cols, int64_t ld, uint64_t seed,
                            uint64_t offset, void *stream);
+
+/*
+ * Fills a column-major device matrix with a_ij = (u - 0.5) * exp(phi * z),
+ * u ~ U[0,1), z ~ N(0,1) (Box-Muller), from the same counter-based stream
+ * (element i uses positions 3i..3i+2).  phi controls the spread of exponents
+ * within rows and columns, the hard case for split-precision GEMM (phi = 0:
+ * uniform; phi = 4: entries spread over ~2^+-17).  Synthetic-input utility
+ * for the split-precision benchmark; no counterpart in the reference.
+ */
+int cosma_amd_fill_lognormal(double *x, int64_t rows, int64_t cols, int64_t ld, uint64_t seed,
+                             double phi, void *stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,216 @@
+"""Split-precision (Ozaki) DGEMM on the INT8 matrix cores, through the C ABI.
+
+Pinned three ways:
+  * exactness: integer-valued inputs that the power-of-two scaling keeps
+    exact give the exact product, bit for bit, for both methods;
+  * emulation: the moduli method must return the exact product of the
+    P-bit-rounded inputs (computed here with Python integers) to 2 ulp, and
+    the slices method must match a float64 emulation of its digit splitting
+    and recombination bit for bit;
+  * accuracy: against the double-double oracle, the default tolerance gives
+    relative Frobenius error <= 1e-14 for uniform inputs.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import from_dev, gemm_bound, stored_shape, to_dev
+
+pytestmark = pytest.mark.gpu
+
+MODULI = [256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193]
+
+
+def op(X, t):
+    return X if t == "N" else np.asfortranarray(X.T)
+
+
+def run_ozaki(handle, ta, tb, m, n, k, alpha, A, B, beta, C0, method, tol=1e-14, count=0, nan_c=False):
+    import torch
+    dA, lda = to_dev(A)
+    dB, ldb = to_dev(B)
+    dC, ldc = to_dev(C0)
+    if nan_c:
+        dC.fill_(float("nan"))
+    info = handle.dgemm_ozaki(ta, tb, m, n, k, alpha, dA.data_ptr(), lda, dB.data_ptr(), ldb, beta,
+                              dC.data_ptr(), ldc, method=method, tolerance=tol, count=count)
+    torch.cuda.synchronize()
+    return from_dev(dC, m), info
+
+
+def row_exponents(X):
+    """frexp exponent of the max |entry| of each row (0 for zero rows)."""
+    mx = np.abs(X).max(axis=1)
+    return np.where(mx > 0, np.frexp(mx)[1], 0)
+
+
+def emulate_moduli(Aop, Bop, P, alpha):
+    E = row_exponents(Aop)
+    F = row_exponents(Bop.T)
+    Ai = np.rint(np.ldexp(Aop, (P - E)[:, None])).astype(np.int64)
+    Bi = np.rint(np.ldexp(Bop, (P - F)[None, :])).astype(np.int64)
+    Cexact = Ai.astype(object) @ Bi.astype(object)          # Python integers: exact
+    m, n = Cexact.shape
+    out = np.empty((m, n))
+    for i in range(m):
+        for j in range(n):
+            out[i, j] = alpha * math.ldexp(float(Cexact[i, j]), int(E[i] + F[j] - 2 * P))
+    return out
+
+
+def emulate_slices(Aop, Bop, s, alpha):
+    def exps(X):
+        mx = np.abs(X).max(axis=1)
+        f, e = np.frexp(mx)
+        e = np.where(f * 128.0 >= 127.5, e + 1, e)
+        return np.where(mx > 0, e, 0)
+
+    def digits(X, e):
+        r = np.ldexp(X, -e[:, None])
+        out = []
+        for _ in range(s):
+            y = r * 128.0
+            d = np.rint(y)
+            r = y - d
+            out.append(d.astype(np.int64))
+        return out
+
+    E, F = exps(Aop), exps(Bop.T)
+    da, db = digits(Aop, E), digits(Bop.T, F)
+    acc = None
+    for d in range(2, s + 2):
+        tot = 0
+        for t in range(1, s + 1):
+            u = d - t
+            if 1 <= u <= s:
+                tot = tot + da[t - 1] @ db[u - 1].T
+        val = np.ldexp(tot.astype(np.float64), -7 * d)
+        acc = val if acc is None else acc + val
+    return alpha * np.ldexp(acc, E[:, None] + F[None, :])
+
+
+@pytest.mark.parametrize("method,count", [(0, 0), (1, 3)])
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T")])
+def test_integer_inputs_are_exact(handle, method, count, ta, tb):
+    m, n, k = 130, 70, 300
+    rng = np.random.default_rng(1)
+    X = rng.integers(-1024, 1025, (m, k)).astype(np.float64)
+    Y = rng.integers(-1024, 1025, (k, n)).astype(np.float64)
+    A, B = np.asfortranarray(op(X, ta)), np.asfortranarray(op(Y, tb))
+    got, info = run_ozaki(handle, ta, tb, m, n, k, 1.0, A, B, 0.0, np.zeros((m, n), order="F"), method,
+                          count=count, nan_c=True)
+    exact = (X.astype(np.int64) @ Y.astype(np.int64)).astype(np.float64)
+    assert info["method"] == method
+    assert np.array_equal(got, exact)
+
+
+@pytest.mark.parametrize("count", [2, 5, 9, 14, 16])
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "T")])
+def test_moduli_equals_exact_product_of_rounded_inputs(handle, count, ta, tb):
+    m, n, k = 37, 45, 129
+    X, Y = oracle.uniform(m, k, count), oracle.uniform(k, n, 100 + count)
+    X[3, :] *= 2.0 ** 40        # a row with a large exponent
+    Y[:, 7] *= 2.0 ** -30       # a column with a small one
+    X[5, :] = 0.0               # a zero row
+    A, B = np.asfortranarray(op(X, ta)), np.asfortranarray(op(Y, tb))
+    got, info = run_ozaki(handle, ta, tb, m, n, k, -1.3, A, B, 0.0, np.zeros((m, n), order="F"), 0, count=count)
+    assert info["count"] == count
+    ref = emulate_moduli(X, Y, info["bits"], -1.3)
+    diff = np.abs(got - ref)
+    assert (diff <= 2 * np.spacing(np.abs(ref))).all(), diff.max()
+
+
+@pytest.mark.parametrize("s", [1, 2, 4, 7, 8])
+def test_slices_match_emulation_bit_for_bit(handle, s):
+    m, n, k = 41, 33, 200
+    X, Y = oracle.uniform(m, k, 7 + s), oracle.uniform(k, n, 70 + s)
+    X[2, :] *= 2.0 ** 20
+    X[9, :] = 0.0
+    got, info = run_ozaki(handle, "N", "N", m, n, k, 0.75, X, Y, 0.0, np.zeros((m, n), order="F"), 1, count=s)
+    assert info["count"] == s and info["int8_products"] == s * (s + 1) // 2
+    assert np.array_equal(got, emulate_slices(X, Y, s, 0.75))
+
+
+@pytest.mark.parametrize("method", [0, 1])
+@pytest.mark.parametrize("m,n,k,ta,tb", [(256, 256, 1024, "N", "N"), (300, 200, 777, "T", "N"),
+                                          (129, 257, 2048, "N", "T"), (512, 64, 4096, "T", "T")])
+def test_default_tolerance_against_double_double(handle, method, m, n, k, ta, tb):
+    A = oracle.uniform(*stored_shape(ta, m, k), 1)
+    B = oracle.uniform(*stored_shape(tb, k, n), 2)
+    C0 = oracle.uniform(m, n, 3)
+    got, info = run_ozaki(handle, ta, tb, m, n, k, 1.0, A, B, 0.5, C0, method)
+    ref = C0.copy(order="F")
+    oracle.dgemm(ta, tb, m, n, k, 1.0, A, A.shape[0], B, B.shape[0], 0.5, ref, m, dd=True)
+    err = oracle.rel_fro(got, ref)
+    assert err <= 1e-14, (err, info)
+    assert err <= info["error_bound"], (err, info)
+
+
+def test_count_from_tolerance_is_monotone(handle):
+    m = n = 64
+    k = 256
+    A, B = oracle.uniform(m, k, 1), oracle.uniform(k, n, 2)
+    counts, errs = [], []
+    for tol in (1e-4, 1e-8, 1e-12, 1e-15):
+        got, info = run_ozaki(handle, "N", "N", m, n, k, 1.0, A, B, 0.0, np.zeros((m, n), order="F"), 0, tol=tol)
+        ref = np.zeros((m, n), order="F")
+        oracle.dgemm("N", "N", m, n, k, 1.0, A, m, B, k, 0.0, ref, m, dd=True)
+        counts.append(info["count"])
+        errs.append(oracle.rel_fro(got, ref))
+        assert errs[-1] <= tol, (tol, errs[-1], info)
+    assert counts == sorted(counts)
+
+
+def test_lognormal_inputs_error_reported(handle):
+    """phi = 4 inputs: entries spread over ~2^+-17 within rows; the error is
+    reported (not bounded by 1e-14 for few moduli), and more moduli reduce it."""
+    import torch
+    import cosma_amd as ca
+    m = n = 128
+    k = 512
+    A = torch.empty((k, m), dtype=torch.float64, device="cuda")
+    B = torch.empty((n, k), dtype=torch.float64, device="cuda")
+    ca.fill_lognormal(A.data_ptr(), m, k, m, 5, 4.0)
+    ca.fill_lognormal(B.data_ptr(), k, n, k, 6, 4.0)
+    torch.cuda.synchronize()
+    Ah, Bh = from_dev(A, m), from_dev(B, k)
+    ref = np.zeros((m, n), order="F")
+    oracle.dgemm("N", "N", m, n, k, 1.0, Ah, m, Bh, k, 0.0, ref, m, dd=True)
+    errs = []
+    for count in (8, 12, 16):
+        got, info = run_ozaki(handle, "N", "N", m, n, k, 1.0, Ah, Bh, 0.0, np.zeros((m, n), order="F"), 0,
+                              count=count)
+        errs.append(oracle.rel_fro(got, ref))
+    assert errs[0] > errs[1] > errs[2]
+    assert errs[2] < 1e-13
+
+
+def test_nonfinite_inputs_fall_back_to_fp64(handle):
+    m, n, k = 40, 30, 50
+    A, B = oracle.uniform(m, k, 1), oracle.uniform(k, n, 2)
+    A[3, 4] = np.inf
+    got, info = run_ozaki(handle, "N", "N", m, n, k, 1.0, A, B, 0.0, np.zeros((m, n), order="F"), 0)
+    assert info["method"] == -1
+    assert np.isinf(got[3]).any() or np.isnan(got[3]).any()
+    assert np.isfinite(np.delete(got, 3, axis=0)).all()
+
+
+def test_semantics_beta_and_degenerate(handle):
+    m, n, k = 50, 40, 64
+    A, B, C0 = oracle.uniform(m, k, 1), oracle.uniform(k, n, 2), oracle.uniform(m, n, 3)
+    got, _ = run_ozaki(handle, "N", "N", m, n, k, 1.0, A, B, 0.0, C0, 0, nan_c=True)
+    assert np.isfinite(got).all()
+    got, _ = run_ozaki(handle, "N", "N", m, n, 0, 1.0, A, B, 0.25, C0, 0)
+    assert np.array_equal(got, 0.25 * C0)
+    got, _ = run_ozaki(handle, "N", "N", m, n, k, 0.0, A, B, 2.0, C0, 1)
+    assert np.array_equal(got, 2.0 * C0)
+
+
+def test_invalid_arguments(handle):
+    import cosma_amd as ca
+    with pytest.raises(ca.CosmaAmdError, match="method"):
+        handle.dgemm_ozaki("N", "N", 4, 4, 4, 1.0, 16, 4, 16, 4, 0.0, 16, 4, method=5)
+    with pytest.raises(ca.CosmaAmdError, match="moduli"):
+        handle.dgemm_ozaki("N", "N", 4, 4, 4, 1.0, 16, 4, 16, 4, 0.0, 16, 4, method=0, count=17)
