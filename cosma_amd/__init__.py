@@ -39,6 +39,20 @@ OZAKI_MODULI = 0
 OZAKI_SLICES = 1
 
 
+class Block(ctypes.Structure):
+    """One local block of a grid layout (cosma_amd_block)."""
+    _fields_ = [("data", c_vp), ("ld", c_i64), ("row", c_i64), ("col", c_i64),
+                ("rows", c_i64), ("cols", c_i64)]
+
+
+def blocks(items):
+    """ctypes array of Block from (data_ptr, ld, row, col, rows, cols) tuples."""
+    arr = (Block * max(1, len(items)))()
+    for i, it in enumerate(items):
+        arr[i] = Block(*it)
+    return arr, len(items)
+
+
 class OzakiInfo(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int), ("count", ctypes.c_int), ("bits", ctypes.c_int),
                 ("int8_products", ctypes.c_int64), ("error_bound", ctypes.c_double)]
@@ -67,6 +81,10 @@ SIGNATURES = {
     "cosma_amd_handle_last_mode": (ctypes.c_int, [c_vp]),
     "cosma_amd_fill_lognormal": (ctypes.c_int, [c_vp, c_i64, c_i64, c_i64, ctypes.c_uint64, c_dbl, c_vp]),
     "cosma_amd_dgemm_set_tuning": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    "cosma_amd_host_alloc": (ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(c_vp)]),
+    "cosma_amd_host_free": (ctypes.c_int, [c_vp]),
+    "cosma_amd_dgemm_layout": (ctypes.c_int, [c_vp, c_char, c_char, c_i64, c_i64, c_i64, c_dbl,
+                                              c_vp, c_i64, c_vp, c_i64, c_dbl, c_vp, c_i64]),
     "cosma_amd_block_transform": (ctypes.c_int, [c_vp, c_char, c_i64, c_i64, c_dbl, c_vp, c_i64,
                                                  c_dbl, c_vp, c_i64, c_vp]),
 }
@@ -163,6 +181,15 @@ class Handle:
 
     def last_mode(self):
         return lib().cosma_amd_handle_last_mode(self.ptr)
+
+    def dgemm_layout(self, transa, transb, m, n, k, alpha, a_blocks, b_blocks, beta, c_blocks):
+        """Grid-layout multiply; *_blocks are lists of (ptr, ld, row, col, rows, cols)."""
+        A, na = blocks(a_blocks)
+        B, nb = blocks(b_blocks)
+        C, nc = blocks(c_blocks)
+        check(lib().cosma_amd_dgemm_layout(self.ptr, _ch(transa), _ch(transb), m, n, k, alpha,
+                                           ctypes.cast(A, c_vp), na, ctypes.cast(B, c_vp), nb, beta,
+                                           ctypes.cast(C, c_vp), nc))
 
     def block_transform(self, op, rows, cols, alpha, src, ld_src, beta, dst, ld_dst, stream=None):
         return check(lib().cosma_amd_block_transform(self.ptr, _ch(op), rows, cols, alpha, src, ld_src,

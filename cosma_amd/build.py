@@ -1,8 +1,14 @@
-"""Builds libcosma_amd.so (gfx950 HIP kernels + C ABI + C++ front-end).
+"""Builds the cosma_amd libraries.
 
-Compiles every source under cosma_amd/csrc with hipcc for gfx950 and links
-them into cosma_amd/libcosma_amd.so.  Object files go to cosma_amd/_build/.
-Run directly (`python -m cosma_amd.build`) or through __graft_entry__.build().
+  libcosma_amd.so         the gfx950 HIP kernels and the C ABI (csrc/*.hip,
+                          compiled by hipcc) plus the C++ COSMA interface
+                          (csrc/front/*.cpp except interpose.cpp, host code
+                          compiled by g++);
+  libcosma_amd_pxgemm.so  ScaLAPACK's p?gemm symbols (csrc/front/interpose.cpp),
+                          linked against libcosma_amd.so.
+
+Object files go to cosma_amd/_build/.  Run directly (`python -m cosma_amd.build`)
+or through __graft_entry__.build().
 """
 import os
 import subprocess
@@ -13,11 +19,16 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
+FRONT = os.path.join(CSRC, "front")
 LIB = os.path.join(HERE, "libcosma_amd.so")
+PXGEMM_LIB = os.path.join(HERE, "libcosma_amd_pxgemm.so")
+INTERPOSE = os.path.join(FRONT, "interpose.cpp")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX", "g++")
 ARCH = "gfx950"
-# Packed-FP32 instructions are disabled for all device code.
+# Packed-FP32 instructions (v_pk_*_f32) are disabled for all device code: some
+# of their forms have produced wrong results on this hardware.
 COMMON = [
     "-std=c++17", "-O3", "-fPIC", f"--offload-arch={ARCH}",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -26,19 +37,22 @@ COMMON = [
 ]
 
 
+HOST = ["-std=c++17", "-O2", "-fPIC", "-Wall", "-I", os.path.join(ROOT, "include"), "-I", FRONT]
+
+
 def _sources():
-    out = []
-    for name in sorted(os.listdir(CSRC)):
-        if name.endswith((".hip", ".cpp")):
-            out.append(os.path.join(CSRC, name))
+    out = [os.path.join(CSRC, n) for n in sorted(os.listdir(CSRC)) if n.endswith(".hip")]
+    out += [os.path.join(FRONT, n) for n in sorted(os.listdir(FRONT))
+            if n.endswith(".cpp") and os.path.join(FRONT, n) != INTERPOSE]
     return out
 
 
 def _compile(src):
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
-    cmd = [HIPCC, *COMMON, "-c", src, "-o", obj]
-    if src.endswith(".cpp"):
-        cmd = [HIPCC, *COMMON, "-x", "hip", "-c", src, "-o", obj]
+    if src.endswith(".hip"):
+        cmd = [HIPCC, *COMMON, "-c", src, "-o", obj]
+    else:
+        cmd = [CXX, *HOST, "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
@@ -51,16 +65,42 @@ def build(verbose=False):
     with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-ldl"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    tmp = PXGEMM_LIB + ".tmp"
+    cmd = [CXX, *HOST, "-shared", "-o", tmp, INTERPOSE, "-L", HERE, "-l:libcosma_amd.so",
+           "-Wl,-rpath,$ORIGIN", "-ldl"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, PXGEMM_LIB)
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {PXGEMM_LIB}")
     return LIB
+
+
+TEST_DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "front_test.cpp")
+TEST_DRIVER = os.path.join(ROOT, "tests", "cpp", "front_test")
+
+
+def build_test_driver(verbose=False):
+    """Links tests/cpp/front_test (the C++ interface's test driver) against libcosma_amd.so."""
+    tmp = TEST_DRIVER + ".tmp"
+    cmd = [CXX, *HOST, "-o", tmp, TEST_DRIVER_SRC, "-L", HERE, "-l:libcosma_amd.so",
+           "-Wl,-rpath,$ORIGIN/../../cosma_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, TEST_DRIVER)
+    if verbose:
+        print(f"built {TEST_DRIVER}")
+    return TEST_DRIVER
 
 
 if __name__ == "__main__":
     build(verbose=True)
+    build_test_driver(verbose=True)
     sys.exit(0)

@@ -216,6 +216,42 @@ int cosma_amd_block_transform(cosma_amd_handle_t h, char op, int64_t rows, int64
     return launch_block_transform(o, rows, cols, alpha, src, ld_src, beta, dst, ld_dst, pick_stream(h, stream));
 }
 
+int cosma_amd_dgemm_layout(cosma_amd_handle_t h, char transa, char transb, int64_t m, int64_t n, int64_t k,
+                           double alpha, const cosma_amd_block *a_blocks, int64_t n_a_blocks,
+                           const cosma_amd_block *b_blocks, int64_t n_b_blocks, double beta,
+                           const cosma_amd_block *c_blocks, int64_t n_c_blocks) {
+    clear_error();
+    CA_CHECK(h != nullptr, "dgemm_layout: null handle");
+    const char ta = norm_trans(transa), tb = norm_trans(transb);
+    CA_CHECK(ta != 0, "dgemm_layout: invalid transa '%c'", transa);
+    CA_CHECK(tb != 0, "dgemm_layout: invalid transb '%c'", transb);
+    CA_CHECK(m >= 0 && n >= 0 && k >= 0, "dgemm_layout: negative dimension");
+    CA_HIP(hipSetDevice(h->device));
+    return layout_dgemm(h, ta, tb, m, n, k, alpha, a_blocks, n_a_blocks, b_blocks, n_b_blocks, beta, c_blocks,
+                        n_c_blocks);
+}
+
+int cosma_amd_host_alloc(size_t bytes, void **ptr) {
+    clear_error();
+    CA_CHECK(ptr != nullptr, "host_alloc: null output pointer");
+    *ptr = nullptr;
+    if (bytes == 0) return COSMA_AMD_OK;
+    hipError_t e = hipHostMalloc(ptr, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *ptr = nullptr;
+        set_error("host_alloc: hipHostMalloc of %zu bytes failed (%s)", bytes, hipGetErrorString(e));
+        return COSMA_AMD_ERR_ALLOC;
+    }
+    return COSMA_AMD_OK;
+}
+
+int cosma_amd_host_free(void *ptr) {
+    clear_error();
+    if (ptr) CA_HIP(hipHostFree(ptr));
+    return COSMA_AMD_OK;
+}
+
 int cosma_amd_fill_uniform(double *x, int64_t rows, int64_t cols, int64_t ld, uint64_t seed,
                            uint64_t offset, void *stream) {
     clear_error();

@@ -80,6 +80,7 @@ struct cosma_amd_handle_s {
     // host-streaming state
     int64_t mem_limit = 0;              // device bytes the streamer may use (0 = 80% of free)
     cosma_amd::DevBuf dA, dB, dC;       // resident operand copies / ring slots
+    cosma_amd::DevBuf dX;               // product before the layout epilogue
     cosma_amd::DevBuf ozaki_ws;         // split-precision workspace
     std::vector<hipEvent_t> events;     // event pool
     size_t events_used = 0;
@@ -107,6 +108,9 @@ int stream_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, i
                  double alpha, const double *a, int64_t lda, const double *b, int64_t ldb,
                  double beta, double *c, int64_t ldc, bool copy_c_back);
 void release_stream_state(cosma_amd_handle_t h);
+int layout_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha,
+                 const cosma_amd_block *A, int64_t na, const cosma_amd_block *B, int64_t nb, double beta,
+                 const cosma_amd_block *C, int64_t nc);
 int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha,
                 const double *a, int64_t lda, const double *b, int64_t ldb, double beta, double *c,
                 int64_t ldc, int method, double tol, int count, cosma_amd_ozaki_info *info, hipStream_t s);

@@ -5,7 +5,7 @@
 namespace cosma_amd {
 namespace {
 
-constexpr int BM = 256, BN = 256, BK = 64, NT = 256;
+constexpr int BM = 256, BN = 256, BK = 128, NT = 256;
 constexpr int NXCD = 8;
 constexpr int GROUP_M = 4;
 
@@ -28,13 +28,12 @@ __device__ __forceinline__ int sym_mod(int x, int m, float inv_m) {
 
 template <int EPI>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void i8gemm_kernel(I8Problem p, I8Jobs jobs) {
-    // LDS tiles: rows of 64 bytes (one k-step), unpadded, with the four
-    // 16-byte chunks of row r stored in the order chunk ^ ((r >> 1) & 3).
-    // Stores (four consecutive lanes per row, matching 64-byte global runs)
-    // fill a contiguous 1 KB per wave instruction, and the MFMA fragment read
-    // (lane 16g + i: row i, chunk g) is free of bank conflicts on gfx950
-    // (measured with SQ_LDS_BANK_CONFLICT; padded rows cost ~1.3 conflict
-    // cycles per read and 2.6 per store).
+    // LDS: two stages of 256 x 128-byte rows per operand (128 KB in all).
+    // Row r holds its eight 16-byte chunks in the order chunk ^ (r & 7): the
+    // stores (eight consecutive lanes per row, one full 128-byte cache line
+    // per row from global memory) fill a contiguous 1 KB per wave
+    // instruction, and each group of eight lanes of a fragment read (rows
+    // r..r+7, one chunk) touches eight different 16-byte bank groups.
     __shared__ __attribute__((aligned(16))) i32x4 sa[2][BM * BK / 16];
     __shared__ __attribute__((aligned(16))) i32x4 sb[2][BN * BK / 16];
 
@@ -72,79 +71,89 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void
         pb_bits |= (uint64_t)jobs.pb[z][q] << (8 * q);
     }
 
-    // global -> register staging: 4 x 16 bytes of A and of B per thread and
-    // step, two sets so that a step's loads are issued two steps before they
-    // are written to LDS (step t uses set t & 1)
-    i32x4 ra[2][4], rb[2][4];
-    auto load_step = [&](int step, int set) {
+    // global -> register staging: 8 x 16 bytes of A and of B per thread and
+    // step.  Addresses are a wave-uniform base (slice, tile, k0) plus a 32-bit
+    // per-thread offset (row (t >> 3) + 32 q, chunk t & 7), so each load needs
+    // one offset register instead of a 64-bit address.
+    i32x4 ra[8], rb[8];
+    const uint32_t lda32 = (uint32_t)p.lda, ldb32 = (uint32_t)p.ldb;
+    const uint32_t offa = (uint32_t)(t >> 3) * lda32 + (uint32_t)(t & 7) * 16u;
+    const uint32_t offb = (uint32_t)(t >> 3) * ldb32 + (uint32_t)(t & 7) * 16u;
+    auto load_step = [&](int step) {
         const int pair = step / ksteps;
         const int64_t k0 = (int64_t)(step - pair * ksteps) * BK;
-        const int8_t *A = p.a + (int64_t)((pa_bits >> (8 * pair)) & 0xff) * p.a_stride;
-        const int8_t *B = p.b + (int64_t)((pb_bits >> (8 * pair)) & 0xff) * p.b_stride;
+        const int8_t *A = p.a + (int64_t)((pa_bits >> (8 * pair)) & 0xff) * p.a_stride + m0 * p.lda + k0;
+        const int8_t *B = p.b + (int64_t)((pb_bits >> (8 * pair)) & 0xff) * p.b_stride + n0 * p.ldb + k0;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int idx = t + NT * q;
-            const int row = idx >> 2, chunk = idx & 3;
-            ra[set][q] = *reinterpret_cast<const i32x4 *>(A + (m0 + row) * p.lda + k0 + chunk * 16);
-            rb[set][q] = *reinterpret_cast<const i32x4 *>(B + (n0 + row) * p.ldb + k0 + chunk * 16);
+        for (int q = 0; q < 8; ++q) {
+            ra[q] = *reinterpret_cast<const i32x4 *>(A + (uint64_t)(offa + (uint32_t)q * 32u * lda32));
+            rb[q] = *reinterpret_cast<const i32x4 *>(B + (uint64_t)(offb + (uint32_t)q * 32u * ldb32));
         }
     };
-    auto swz = [](int row, int chunk) { return row * 4 + (chunk ^ ((row >> 1) & 3)); };
-    auto store_step = [&](int st, int set) {
+    auto swz = [](int row, int chunk) { return row * 8 + (chunk ^ (row & 7)); };
+    auto store_step = [&](int st) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < 8; ++q) {
             const int idx = t + NT * q;
-            const int row = idx >> 2, chunk = idx & 3;
-            sa[st][swz(row, chunk)] = ra[set][q];
-            sb[st][swz(row, chunk)] = rb[set][q];
+            const int row = idx >> 3, chunk = idx & 7;
+            sa[st][swz(row, chunk)] = ra[q];
+            sb[st][swz(row, chunk)] = rb[q];
         }
     };
 
-    // Fragment registers, double-buffered across k-steps: while the MFMAs of
-    // step s run on F[s & 1], the fragments of step s + 1 are read from the
-    // other LDS stage into F[(s + 1) & 1].
+    // Fragments of the two 64-deep halves of a step: F[0] (bytes 0..63 of
+    // the rows) and F[1] (bytes 64..127).
     i32x4 fa[2][8], fb[2][8];
-    auto read_frags = [&](int st) {
+    auto read_frags = [&](int st, int h) {
 #pragma unroll
-        for (int j = 0; j < 8; ++j) fb[st][j] = sb[st][swz(wn * 128 + 16 * j + li, g)];
+        for (int j = 0; j < 8; ++j) fb[h][j] = sb[st][swz(wn * 128 + 16 * j + li, 4 * h + g)];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) fa[st][i] = sa[st][swz(wm * 128 + 16 * i + li, g)];
+        for (int i = 0; i < 8; ++i) fa[h][i] = sa[st][swz(wm * 128 + 16 * i + li, 4 * h + g)];
     };
-    auto mfmas = [&](int st) {
+    auto mfmas = [&](int h) {
 #pragma unroll
         for (int i = 0; i < 8; ++i)
 #pragma unroll
             for (int j = 0; j < 8; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[st][i], fb[st][j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[h][i], fb[h][j], acc[i][j], 0, 0, 0);
     };
-    // Step s lives in LDS stage s & 1.  Iteration s: read the fragments of step
-    // s + 1, run the MFMAs of step s, write step s + 2 (register set s & 1)
-    // into stage s & 1 (its fragments were read before the previous barrier),
-    // load step s + 4 into the register set just freed, barrier.
-    // The body is branch-free (loads past the end re-read the last step, and
-    // the extra stores and fragment reads are never consumed) so that the
-    // compiler's wait counts only cover the set being stored: a guarded body
-    // made it wait for the newer prefetch set as well, halving the prefetch
-    // distance.  nsteps is even (K is a multiple of 128).
+    // Step s lives in LDS stage s & 1.  Entering iteration s, the registers
+    // hold the global data of step s + 1 and F[0] holds the first half of
+    // step s.  Iteration s: read the second half of step s into F[1], run
+    // both halves' MFMAs, store step s + 1 into the other stage (whose last
+    // reads happened before the previous barrier), load step s + 2, barrier,
+    // read the first half of step s + 1.  The loads of a step are issued one
+    // full iteration (128 MFMAs) before they are stored; the scheduler
+    // spreads the stores and loads between the MFMAs.  The body is
+    // branch-free (loads past the end re-read the last step; the extra store
+    // and fragment reads are never consumed), so the compiler's wait counts
+    // stay exact; nsteps is even (K is a multiple of 256), so the stage
+    // indices are constants.  The scheduling barriers keep each batch of
+    // fragment reads together ahead of the MFMAs: left alone, the scheduler
+    // sinks every A-fragment read to just before its first use and waits for
+    // it there.  (Fencing the stores and loads off as a block between the two
+    // halves, with the barrier in the middle of the iteration, measured 12%
+    // slower.)
     const int last = nsteps - 1;
     auto iteration = [&](int s, int st) {
-        read_frags(st ^ 1);
-        mfmas(st);
-        store_step(st, st);
-        load_step(min(s + 4, last), st);
+        read_frags(st, 1);
+        __builtin_amdgcn_sched_barrier(0);
+        mfmas(0);
+        mfmas(1);
+        store_step(st ^ 1);
+        load_step(min(s + 2, last));
         __syncthreads();
+        read_frags(st ^ 1, 0);
+        __builtin_amdgcn_sched_barrier(0);
     };
 
     if (nsteps > 0) {
-        load_step(0, 0);
-        load_step(min(1, last), 1);
-        store_step(0, 0);
-        load_step(min(2, last), 0);
+        load_step(0);
+        store_step(0);
+        load_step(min(1, last));
         __syncthreads();
-        read_frags(0);
-        store_step(1, 1);
-        load_step(min(3, last), 1);
-        __syncthreads();
+        read_frags(0, 0);
+        __builtin_amdgcn_sched_barrier(0);
         for (int s = 0; s < nsteps; s += 2) {
             iteration(s, 0);
             iteration(s + 1, 1);
@@ -166,11 +175,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void
             const i32x4 r = acc[i][j];
             const int64_t off = (int64_t)z * p.out_stride + row + col * p.ldo;
             const bool full = row + 3 < p.M;
-            if (EPI == EPI_I32) {
-                int *o = static_cast<int *>(p.out) + off;
-                if (full) *reinterpret_cast<i32x4 *>(o) = r;
-                else for (int e = 0; e < 4 && row + e < p.M; ++e) o[e] = r[e];
-            } else if (EPI == EPI_RES8) {
+            if (EPI == EPI_RES8) {
                 int8_t *o = static_cast<int8_t *>(p.out) + off;
                 int8_t v[4];
                 for (int e = 0; e < 4; ++e) v[e] = (int8_t)sym_mod(r[e], mod, inv_m);
@@ -200,11 +205,15 @@ int launch_i8gemm(const I8Problem &p, const I8Jobs &jobs, int Z, I8Epilogue epi,
         return COSMA_AMD_ERR_UNSUPPORTED;
     }
     auto a16 = [](const void *x) { return (reinterpret_cast<uintptr_t>(x) & 15) == 0; };
+    if ((p.lda * BM) >> 31 || (p.ldb * BN) >> 31) {
+        set_error("i8gemm: leading dimensions of %lld / %lld bytes are too large", (long long)p.lda, (long long)p.ldb);
+        return COSMA_AMD_ERR_UNSUPPORTED;
+    }
     if (p.K % (2 * BK) || p.lda % 16 || p.ldb % 16 || p.a_stride % 16 || p.b_stride % 16 || !a16(p.a) || !a16(p.b)) {
-        set_error("i8gemm: K must be a multiple of 128 and strides/bases 16-byte aligned");
+        set_error("i8gemm: K must be a multiple of 256 and strides/bases 16-byte aligned");
         return COSMA_AMD_ERR_INVALID_ARG;
     }
-    if ((epi == EPI_I32 || epi == EPI_RES8) && (p.ldo % 4 || p.out_stride % 4 ||
+    if (epi == EPI_RES8 && (p.ldo % 4 || p.out_stride % 4 ||
                                                  (reinterpret_cast<uintptr_t>(p.out) & 3))) {
         set_error("i8gemm: output leading dimension must be a multiple of 4");
         return COSMA_AMD_ERR_INVALID_ARG;
@@ -222,7 +231,6 @@ int launch_i8gemm(const I8Problem &p, const I8Jobs &jobs, int Z, I8Epilogue epi,
     const int64_t tiles = ((p.M + BM - 1) / BM) * ((p.N + BN - 1) / BN);
     const dim3 grid((unsigned)tiles, (unsigned)Z), block(NT);
     switch (epi) {
-    case EPI_I32: hipLaunchKernelGGL(i8gemm_kernel<EPI_I32>, grid, block, 0, s, p, jobs); break;
     case EPI_RES8: hipLaunchKernelGGL(i8gemm_kernel<EPI_RES8>, grid, block, 0, s, p, jobs); break;
     default: hipLaunchKernelGGL(i8gemm_kernel<EPI_F64ACC>, grid, block, 0, s, p, jobs); break;
     }

@@ -11,7 +11,6 @@
 //     acc_z = sum over the pairs p of job z of  A[pa[z][p]] * B[pb[z][p]]^T
 // (a pair list concatenates products along k without moving data), and hands
 // acc_z to an epilogue:
-//   EPI_I32     store the INT32 result (tests);
 //   EPI_RES8    reduce modulo the job's modulus into the symmetric range and
 //               store one int8 residue per element (moduli method);
 //   EPI_F64ACC  C64 = scale * acc (+ C64 if accumulate) in FP64 (slices method;
@@ -19,9 +18,10 @@
 //
 // Tile: 256 x 256 per workgroup of 256 threads (2 x 2 waves of 128 x 128, i.e.
 // 8 x 8 MFMA blocks of 16 x 16 per wave, 256 INT32 accumulators per lane in
-// AGPRs), k-step 64 (one MFMA depth).  LDS tiles use unpadded 64-byte rows
-// with a chunk swizzle (chunk ^ ((row >> 1) & 3)) that keeps the fragment
-// reads and the stores free of bank conflicts.
+// AGPRs), k-step 128 (two MFMA depths: every row of a global load is one
+// full 128-byte cache line).  LDS tiles use unpadded 128-byte rows with a
+// chunk swizzle (chunk ^ (row & 7)) for conflict-free stores and fragment
+// reads; two LDS stages (128 KB) double-buffer the steps.
 // MFMA i32 16x16x64 lane layout (checked on gfx950): lane l = 16g + i supplies
 // A row i / B column i, bytes k = 16g .. 16g+15; it receives D rows 4g..4g+3 of
 // column i.
@@ -33,7 +33,7 @@ namespace cosma_amd {
 constexpr int I8_MAX_PAIRS = 8;
 constexpr int I8_MAX_JOBS = 24;
 
-enum I8Epilogue { EPI_I32 = 0, EPI_RES8 = 1, EPI_F64ACC = 2 };
+enum I8Epilogue { EPI_RES8 = 1, EPI_F64ACC = 2 };
 
 struct I8Jobs {
     int npairs[I8_MAX_JOBS];
@@ -57,9 +57,9 @@ struct I8Problem {
     int accumulate;       // EPI_F64ACC: add into the output
 };
 
-// Requirements: K % 128 == 0; lda, ldb, a_stride, b_stride multiples of 16 and
-// 16-byte aligned bases; int8/int32 outputs with ldo and out_stride multiples
-// of 4 (all checked).  Not checkable, hence the callers' duty: every A slice
+// Requirements: K % 256 == 0; lda, ldb, a_stride, b_stride multiples of 16 and
+// 16-byte aligned bases; int8 outputs with ldo and out_stride multiples of 4
+// (all checked).  Not checkable, hence the callers' duty: every A slice
 // holds ceil(M/256)*256 rows and every B slice ceil(N/256)*256 columns (the
 // split kernels pad with zeros), so tile loads never need bounds checks.
 int launch_i8gemm(const I8Problem &p, const I8Jobs &jobs, int Z, I8Epilogue epi, hipStream_t s);

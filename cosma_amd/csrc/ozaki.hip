@@ -374,7 +374,7 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
     }
 
     // ---- workspace
-    const int64_t Mp = round_up(m, 256), Np = round_up(n, 256), Kp = round_up(k, 128);
+    const int64_t Mp = round_up(m, 256), Np = round_up(n, 256), Kp = round_up(k, 256);
     const int64_t ldr = round_up(m, 4);                         // residue output ld
     const size_t a_bytes = (size_t)count * Mp * Kp, b_bytes = (size_t)count * Np * Kp;
     const size_t out_bytes = method == COSMA_AMD_OZAKI_MODULI ? (size_t)count * ldr * n

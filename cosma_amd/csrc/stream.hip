@@ -260,7 +260,7 @@ int upload_op_block(cosma_amd_handle_t h, const Operand &X, double *dst, int64_t
 
 void release_stream_state(cosma_amd_handle_t h) {
     hipSetDevice(h->device);
-    for (DevBuf *b : {&h->dA, &h->dB, &h->dC, &h->ozaki_ws}) {
+    for (DevBuf *b : {&h->dA, &h->dB, &h->dC, &h->dX, &h->ozaki_ws}) {
         if (b->ptr) hipFree(b->ptr);
         b->ptr = nullptr;
         b->bytes = 0;
@@ -468,6 +468,184 @@ int stream_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, i
         h->dev_c_ld = ldC;
     }
     return COSMA_AMD_OK;
+}
+
+// ------------------------------------------------------ grid-layout multiply
+namespace {
+
+// Checks that the blocks lie inside a rows x cols matrix, are well formed and
+// tile it exactly: in bounds + total area == rows * cols + no two blocks
+// overlap (sorted sweep by column start, then row start, per column band).
+int check_tiling(const char *name, const cosma_amd_block *blk, int64_t nb, int64_t rows, int64_t cols) {
+    if (nb < 0 || (nb > 0 && !blk)) {
+        set_error("layout multiply: %s has an invalid block list", name);
+        return COSMA_AMD_ERR_INVALID_ARG;
+    }
+    __int128 area = 0;
+    for (int64_t i = 0; i < nb; ++i) {
+        const cosma_amd_block &b = blk[i];
+        if (b.rows < 0 || b.cols < 0 || b.row < 0 || b.col < 0 || b.row + b.rows > rows ||
+            b.col + b.cols > cols) {
+            set_error("layout multiply: block %lld of %s ([%lld, +%lld) x [%lld, +%lld)) is outside the "
+                      "%lld x %lld matrix", (long long)i, name, (long long)b.row, (long long)b.rows,
+                      (long long)b.col, (long long)b.cols, (long long)rows, (long long)cols);
+            return COSMA_AMD_ERR_INVALID_ARG;
+        }
+        if (b.rows > 0 && b.cols > 0 && (!b.data || b.ld < b.rows)) {
+            set_error("layout multiply: block %lld of %s has a null pointer or ld %lld < rows %lld",
+                      (long long)i, name, (long long)b.ld, (long long)b.rows);
+            return COSMA_AMD_ERR_INVALID_ARG;
+        }
+        area += (__int128)b.rows * b.cols;
+    }
+    if (area != (__int128)rows * cols) {
+        set_error("layout multiply: the blocks of %s cover %lld elements, the %lld x %lld matrix has %lld",
+                  name, (long long)area, (long long)rows, (long long)cols, (long long)(rows * cols));
+        return COSMA_AMD_ERR_INVALID_ARG;
+    }
+    // Equal total area plus no overlap means an exact tiling.
+    std::vector<int64_t> order;
+    order.reserve(nb);
+    for (int64_t i = 0; i < nb; ++i)
+        if (blk[i].rows > 0 && blk[i].cols > 0) order.push_back(i);
+    std::sort(order.begin(), order.end(), [&](int64_t x, int64_t y) {
+        return blk[x].row != blk[y].row ? blk[x].row < blk[y].row : blk[x].col < blk[y].col;
+    });
+    for (size_t p = 0; p < order.size(); ++p) {
+        const cosma_amd_block &x = blk[order[p]];
+        for (size_t q = p + 1; q < order.size(); ++q) {
+            const cosma_amd_block &y = blk[order[q]];
+            if (y.row >= x.row + x.rows) break;   // sorted by row start: no later block overlaps x
+            if (y.col < x.col + x.cols && x.col < y.col + y.cols) {
+                set_error("layout multiply: blocks %lld and %lld of %s overlap", (long long)order[p],
+                          (long long)order[q], name);
+                return COSMA_AMD_ERR_INVALID_ARG;
+            }
+        }
+    }
+    return COSMA_AMD_OK;
+}
+
+// Gathers the blocks of one operand into the column-major device matrix dst
+// (leading dimension ld): host blocks are uploaded on h2d, device blocks are
+// copied by the block-copy kernel on stream s.
+int gather_blocks(cosma_amd_handle_t h, const cosma_amd_block *blk, int64_t nb, double *dst, int64_t ld,
+                  hipStream_t s) {
+    for (int64_t i = 0; i < nb; ++i) {
+        const cosma_amd_block &b = blk[i];
+        if (b.rows == 0 || b.cols == 0) continue;
+        const double *src = static_cast<const double *>(b.data);
+        double *d = dst + b.row + b.col * ld;
+        const Kind kind = classify(src);
+        int rc = (kind == DEVICE) ? launch_block_transform('N', b.rows, b.cols, 1.0, src, b.ld, 0.0, d, ld, s)
+                                  : upload(h, d, ld, src, b.ld, b.rows, b.cols, kind);
+        if (rc) return rc;
+    }
+    return COSMA_AMD_OK;
+}
+
+}  // namespace
+
+int layout_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha,
+                 const cosma_amd_block *A, int64_t na, const cosma_amd_block *B, int64_t nb, double beta,
+                 const cosma_amd_block *C, int64_t nc) {
+    const int64_t arows = ta == 'N' ? m : k, acols = ta == 'N' ? k : m;
+    const int64_t brows = tb == 'N' ? k : n, bcols = tb == 'N' ? n : k;
+    int rc;
+    if ((rc = check_tiling("C", C, nc, m, n))) return rc;
+    if (m == 0 || n == 0) return COSMA_AMD_OK;
+    const bool scale_only = (k == 0 || alpha == 0.0);
+    if (!scale_only) {
+        if ((rc = check_tiling("A", A, na, arows, acols))) return rc;
+        if ((rc = check_tiling("B", B, nb, brows, bcols))) return rc;
+    }
+    h->events_used = 0;
+    h->dev_c = nullptr;
+    h->last_mode = 3;
+    hipStream_t s = h->streams[0];
+
+    // ---- device copies: A, B as stored, X = op(A) op(B), staging for host C blocks
+    bool host_c = false;
+    for (int64_t i = 0; i < nc; ++i) host_c |= (C[i].rows > 0 && C[i].cols > 0 && classify(C[i].data) != DEVICE);
+    const size_t needA = scale_only ? 0 : (size_t)arows * acols * 8, needB = scale_only ? 0 : (size_t)brows * bcols * 8;
+    const size_t needX = (size_t)m * n * 8, needC = host_c ? needX : 0;
+    size_t free_b = 0, total_b = 0;
+    CA_HIP(hipMemGetInfo(&free_b, &total_b));
+    const size_t limit = h->mem_limit > 0 ? (size_t)h->mem_limit : (size_t)(0.8 * (double)free_b);
+    if (needA + needB + needX + needC > limit) {
+        set_error("layout multiply: the gathered operands need %zu bytes of device memory, the budget is %zu",
+                  needA + needB + needX + needC, limit);
+        return COSMA_AMD_ERR_UNSUPPORTED;
+    }
+    if ((rc = grow(h->dX, needX))) return rc;
+    double *dX = static_cast<double *>(h->dX.ptr);
+    double *dC = nullptr;
+    if (host_c) {
+        if ((rc = grow(h->dC, needC))) return rc;
+        dC = static_cast<double *>(h->dC.ptr);
+    }
+    TRACE("layout dgemm %c%c m=%lld n=%lld k=%lld blocks %lld/%lld/%lld host C %d", ta, tb, (long long)m,
+          (long long)n, (long long)k, (long long)na, (long long)nb, (long long)nc, (int)host_c);
+
+    // ---- C blocks that live on the host (read only if beta != 0) go up first
+    if (host_c && beta != 0.0) {
+        for (int64_t i = 0; i < nc; ++i) {
+            const cosma_amd_block &b = C[i];
+            if (b.rows == 0 || b.cols == 0) continue;
+            const Kind kind = classify(b.data);
+            if (kind == DEVICE) continue;
+            rc = upload(h, dC + b.row + b.col * m, m, static_cast<const double *>(b.data), b.ld, b.rows, b.cols, kind);
+            if (rc) return rc;
+        }
+    }
+
+    // ---- X = op(A) op(B) with (alpha, beta) = (1, 0), or X = 0 for the scale-only case
+    if (scale_only) {
+        CA_HIP(hipMemsetAsync(dX, 0, needX, s));
+        alpha = 0.0;
+    } else {
+        if ((rc = grow(h->dA, needA)) || (rc = grow(h->dB, needB))) return rc;
+        double *dA = static_cast<double *>(h->dA.ptr), *dB = static_cast<double *>(h->dB.ptr);
+        if ((rc = gather_blocks(h, A, na, dA, arows, s))) return rc;
+        if ((rc = gather_blocks(h, B, nb, dB, brows, s))) return rc;
+        hipEvent_t up;
+        if ((rc = record(h, h->h2d, &up))) return rc;
+        CA_HIP(hipStreamWaitEvent(s, up, 0));
+        if ((rc = launch_dgemm(ta, tb, m, n, k, 1.0, dA, arows, dB, brows, 0.0, dX, m, s))) return rc;
+    }
+    {
+        hipEvent_t up;
+        if ((rc = record(h, h->h2d, &up))) return rc;   // host C blocks are on the device
+        CA_HIP(hipStreamWaitEvent(s, up, 0));
+    }
+
+    // ---- epilogue per C block: C_blk <- alpha * X_blk + beta * C_blk
+    for (int64_t i = 0; i < nc; ++i) {
+        const cosma_amd_block &b = C[i];
+        if (b.rows == 0 || b.cols == 0) continue;
+        const double *x = dX + b.row + b.col * m;
+        if (classify(b.data) == DEVICE) {
+            rc = launch_block_transform('N', b.rows, b.cols, alpha, x, m, beta, static_cast<double *>(b.data),
+                                        b.ld, s);
+        } else {
+            rc = launch_block_transform('N', b.rows, b.cols, alpha, x, m, beta, dC + b.row + b.col * m, m, s);
+        }
+        if (rc) return rc;
+    }
+    if (host_c) {
+        hipEvent_t done;
+        if ((rc = record(h, s, &done))) return rc;
+        CA_HIP(hipStreamWaitEvent(h->d2h, done, 0));
+        for (int64_t i = 0; i < nc; ++i) {
+            const cosma_amd_block &b = C[i];
+            if (b.rows == 0 || b.cols == 0) continue;
+            const Kind kind = classify(b.data);
+            if (kind == DEVICE) continue;
+            rc = download(h, static_cast<double *>(b.data), b.ld, dC + b.row + b.col * m, m, b.rows, b.cols, kind);
+            if (rc) return rc;
+        }
+    }
+    return sync_all(h);
 }
 
 }  // namespace cosma_amd

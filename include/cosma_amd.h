@@ -193,6 +193,56 @@ int cosma_amd_block_transform(cosma_amd_handle_t handle, char op,
                               const double *src, int64_t ld_src, double beta,
                               double *dst, int64_t ld_dst, void *stream);
 
+/* ------------------------------------------------------ pinned host memory */
+/*
+ * Allocates `bytes` of page-locked host memory (hipHostMalloc) that the
+ * host-streaming GEMM copies by DMA without staging; frees it.  The C++
+ * front-end's CosmaMatrix stores its local matrix here.  Replaces the
+ * reference's host memory pool plus in-place registration of its buffers
+ * (memory_pool<T>::pin, src/cosma/memory_pool.cpp:153-165;
+ * pinned_buffers<T>::add, src/cosma/pinned_buffers.cpp:6-47).
+ */
+int cosma_amd_host_alloc(size_t bytes, void **ptr);
+int cosma_amd_host_free(void *ptr);
+
+/* ----------------------------------------------- grid-layout multiply */
+/*
+ * One local block of a matrix given as a grid of blocks: the block covers
+ * stored rows [row, row + rows) and columns [col, col + cols) of the matrix
+ * and its element (i, j) is data[i + j * ld] (host or device memory, like the
+ * operands of cosma_amd_dgemm_host).
+ * Plays the role of COSTA's costa::block<T> built from the C API struct
+ * `block` (src/cosma/cinterface.hpp:16-21, src/cosma/cinterface.cpp:20-31).
+ */
+typedef struct {
+    void *data;
+    int64_t ld;
+    int64_t row, col;
+    int64_t rows, cols;
+} cosma_amd_block;
+
+/*
+ * C <- alpha * op(A) * op(B) + beta * C for matrices that are each given as a
+ * set of blocks that tile the stored matrix exactly (A is m x k if transa is
+ * 'N', else k x m; B likewise; C is m x n).  The blocks are gathered into
+ * device copies (block uploads / the block-copy kernel), the product
+ * X = op(A) * op(B) is formed once on the device with the FP64 MFMA DGEMM,
+ * and every C block is updated as C_blk <- alpha * X_blk + beta * C_blk by
+ * the block-transform kernel (the reference's order: multiply with (1, 0),
+ * then the layout transform applies alpha and beta).  beta == 0 never reads
+ * C; k == 0 or alpha == 0 gives beta * C.  Blocks that overlap, leave a gap,
+ * or fall outside the matrix are rejected.  Returns when C is final.
+ * Replaces the single-rank work of cosma::multiply_using_layout<T>
+ * (src/cosma/multiply.cpp:77-213) behind dmultiply_using_layout
+ * (src/cosma/cinterface.hpp:52-59).
+ */
+int cosma_amd_dgemm_layout(cosma_amd_handle_t handle, char transa, char transb,
+                           int64_t m, int64_t n, int64_t k, double alpha,
+                           const cosma_amd_block *a_blocks, int64_t n_a_blocks,
+                           const cosma_amd_block *b_blocks, int64_t n_b_blocks,
+                           double beta, const cosma_amd_block *c_blocks,
+                           int64_t n_c_blocks);
+
 /* ------------------------------------------------------- synthetic inputs */
 /*
  * Fills a column-major rows x cols device matrix (leading dimension ld) with
