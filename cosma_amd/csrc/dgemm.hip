@@ -229,41 +229,90 @@ __global__ __launch_bounds__(NT) void dgemm_mfma_kernel(
                 acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[h][i].y, bf[h][j].y, acc[i][j], 0, 0, 0);
     };
 
-    const int64_t nk_full = K / BK;
-    const int64_t nk = (K + BK - 1) / BK;
+    // LDS stores per thread and step, MFMAs per k-sub-step
+    constexpr int N_WRITES = (TA ? 1 : 2) * IOA::PER_T + (TB ? 2 : 1) * IOB::PER_T;
+    constexpr int MFMA_HALF = 2 * FM * FN;
+    constexpr int MFMA_PER_WRITE = MFMA_HALF / (N_WRITES + 1);
+    // global loads per thread and step (VEC: 16-byte loads; else 8-byte)
+    constexpr int N_LOADS = (VEC ? 1 : 2) * (IOA::NREG + IOB::NREG);
+    constexpr int MFMA_PER_LOAD = MFMA_HALF / (N_LOADS + 1) > 0 ? MFMA_HALF / (N_LOADS + 1) : 1;
+    const int64_t nk_full = K / BK;          // full k-steps
+    const bool has_tail = (K % BK) != 0;     // one partial step after them
     f64x2 ra[IOA::NREG], rb[IOB::NREG];
-    auto load_step = [&](int64_t kn) {
-        if (kn < nk_full) {
-            IOA::load(A, lda, m0, kn * BK, M, ra);
-            IOB::load(B, ldb, n0, kn * BK, N, rb);
-        } else {                     // the partial step at the end of K
-            IOA::load_tail(A, lda, m0, kn * BK, M, K, ra);
-            IOB::load_tail(B, ldb, n0, kn * BK, N, K, rb);
-        }
+    auto load_full = [&](int64_t kn) {
+        IOA::load(A, lda, m0, kn * BK, M, ra);
+        IOB::load(B, ldb, n0, kn * BK, N, rb);
+    };
+    auto load_tail = [&](int64_t kn) {
+        IOA::load_tail(A, lda, m0, kn * BK, M, K, ra);
+        IOB::load_tail(B, ldb, n0, kn * BK, N, K, rb);
+    };
+    auto store_regs = [&](int st) {
+        IOA::store(sa[st], ra);
+        IOB::store(sb[st], rb);
     };
 
-    // Pipeline: registers hold step kt+1 while step kt is multiplied out of
-    // LDS stage kt&1; halfway through step kt they are written to stage
-    // (kt+1)&1 and refilled with step kt+2 from global memory.
-    load_step(0);
-    IOA::store(sa[0], ra);
-    IOB::store(sb[0], rb);
-    if (nk > 1) load_step(1);
-    __syncthreads();
-    read_frags(0, 0, 0);
-
-    for (int64_t kt = 0; kt < nk; ++kt) {
-        const int st = kt & 1;
+    // Pipeline over the full steps.  Entering iteration kt, LDS stage kt&1
+    // holds step kt, F[0] holds its first k-sub-step and the registers hold
+    // step kt+1.  Iteration kt: read sub-step 1 into F[1]; MFMAs of F[0];
+    // store step kt+1 into the other stage (last read before the previous
+    // barrier); load step kt+2; barrier; read sub-step 0 of step kt+1 into
+    // F[0]; MFMAs of F[1].  Each batch of fragment reads is issued 64 MFMAs
+    // before it is used, and each global load a full step before it is
+    // stored.  The body has no branches (the loads of the last iterations
+    // re-read the last full step; the extra store and fragment reads are
+    // never used), which keeps the compiler's wait counts partial, and the
+    // scheduling barriers stop the scheduler from sinking fragment reads to
+    // their first use.
+    if (nk_full > 0) {
+        load_full(0);
+        store_regs(0);
+        load_full(min((int64_t)1, nk_full - 1));
+        __syncthreads();
+        read_frags(0, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
+        for (int64_t kt = 0; kt < nk_full; ++kt) {
+            const int st = (int)(kt & 1);
+            read_frags(st, 1, 1);
+            __builtin_amdgcn_sched_barrier(0);
+            mfma_frags(0);
+            store_regs(st ^ 1);
+            // spread the LDS stores over the MFMAs instead of stalling the
+            // wave on them after the last MFMA
+#pragma unroll
+            for (int w = 0; w < N_WRITES; ++w) {
+                __builtin_amdgcn_sched_group_barrier(0x008, MFMA_PER_WRITE, 0);   // MFMA
+                __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);                // DS write
+            }
+            __builtin_amdgcn_sched_group_barrier(0x008, MFMA_HALF - N_WRITES * MFMA_PER_WRITE, 0);
+            __builtin_amdgcn_sched_barrier(0);
+            __syncthreads();
+            read_frags(st ^ 1, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
+            mfma_frags(1);
+            load_full(min(kt + 2, nk_full - 1));
+            // and the global loads over the MFMAs of the second sub-step
+#pragma unroll
+            for (int w = 0; w < N_LOADS; ++w) {
+                __builtin_amdgcn_sched_group_barrier(0x008, MFMA_PER_LOAD, 0);    // MFMA
+                __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);                // VMEM read
+            }
+            __builtin_amdgcn_sched_group_barrier(0x008, MFMA_HALF - N_LOADS * MFMA_PER_LOAD, 0);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    if (has_tail) {
+        // The partial last step, zero-filled beyond K, through stage
+        // nk_full & 1 (whose last reads finish before this barrier).
+        load_tail(nk_full);
+        __syncthreads();
+        const int st = (int)(nk_full & 1);
+        store_regs(st);
+        __syncthreads();
+        read_frags(st, 0, 0);
         read_frags(st, 1, 1);
         mfma_frags(0);
-        if (kt + 1 < nk) {
-            IOA::store(sa[st ^ 1], ra);
-            IOB::store(sb[st ^ 1], rb);
-            if (kt + 2 < nk) load_step(kt + 2);
-        }
         mfma_frags(1);
-        __syncthreads();
-        if (kt + 1 < nk) read_frags(st ^ 1, 0, 0);
     }
 
     // ---- epilogue: lane holds rows 4g..4g+3 (contiguous) of column li
