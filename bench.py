@@ -32,12 +32,14 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", default="device", choices=["device", "host", "tall-skinny", "ozaki"],
+    p.add_argument("--workload", default="device",
+                   choices=["device", "host", "tall-skinny", "ozaki", "reorder"],
                    help="device: 16384^3 TN on device-resident operands (default, the headline); "
                         "host: same GEMM on pinned host operands streamed in 4096^2 tiles; "
                         "tall-skinny: m=n=2048, k=1048576 A^T B streamed from pinned host memory "
                         "in k-panels of 65536; ozaki: 8192^3 split-precision vs plain FP64 for "
-                        "phi in {0, 1, 4}")
+                        "phi in {0, 1, 4}; reorder: the layout transforms of a 16384^2 pdgemm "
+                        "(A transposed, B copied, C <- alpha X + beta C) on the device")
     p.add_argument("--no-split", action="store_true", help="skip the split-precision summary leg")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     p.add_argument("--cpu-n", type=int, default=8192, help="CPU sample size (m = n = k)")
@@ -254,6 +256,58 @@ def split_precision(phis=(0.0, 1.0, 4.0), steps=5, warmup=1, n=8192, rows=(0, 40
     return results
 
 
+PEAK_HBM = 8000.0  # GB/s, MI355X datasheet
+
+
+def reorder(args):
+    """Device block transforms of the ScaLAPACK -> COSMA -> ScaLAPACK round trip
+    of a 16384^2 pdgemm on one rank: A transposed (op = 'T'), B copied
+    (op = 'N'), the result written back as C <- alpha * X + beta * C."""
+    import torch
+    import cosma_amd as ca
+    n = 16384
+    h = ca.Handle(0)
+    X = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    Y = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    Z = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    ca.fill_uniform(X.data_ptr(), n, n, n, 31)
+    ca.fill_uniform(Y.data_ptr(), n, n, n, 32)
+    ca.fill_uniform(Z.data_ptr(), n, n, n, 33)
+    torch.cuda.synchronize()
+    nb = 8.0 * n * n
+    legs = {"A transpose (alpha=1, beta=0)": (lambda: h.block_transform("T", n, n, 1.0, X.data_ptr(), n, 0.0,
+                                                                          Y.data_ptr(), n), 2 * nb),
+            "B copy (alpha=1, beta=0)": (lambda: h.block_transform("N", n, n, 1.0, X.data_ptr(), n, 0.0,
+                                                                     Y.data_ptr(), n), 2 * nb),
+            "C epilogue (alpha=1, beta=0.5)": (lambda: h.block_transform("N", n, n, 1.0, X.data_ptr(), n, 0.5,
+                                                                           Z.data_ptr(), n), 3 * nb)}
+    res, total_b, total_t = {}, 0.0, 0.0
+    for name, (fn, nbytes) in legs.items():
+        for _ in range(args.warmup):
+            fn()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        s.record()
+        for _ in range(args.steps):
+            fn()
+        e.record()
+        torch.cuda.synchronize()
+        ms = s.elapsed_time(e) / args.steps
+        res[name] = {"ms": round(ms, 4), "GB/s": round(nbytes / ms / 1e6, 1),
+                     "frac_of_hbm": round(nbytes / ms / 1e6 / PEAK_HBM, 4)}
+        total_b += nbytes
+        total_t += ms
+    out = {"metric": "device layout-transform bandwidth (algorithmic bytes / time)",
+           "value": round(total_b / total_t / 1e6, 1), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(total_t, 4), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded uniform)",
+           "config": {"workload": "16384^2 ScaLAPACK <-> COSMA transforms of one pdgemm on one rank"},
+           "legs": res,
+           "roofline": {"bound": "hbm", "achieved": round(total_b / total_t / 1e6, 1), "peak": PEAK_HBM,
+                        "unit": "GB/s", "frac": round(total_b / total_t / 1e6 / PEAK_HBM, 4), "traffic": None}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -271,6 +325,8 @@ def main():
                "error_note": "rel_err = relative Frobenius error of 2 sampled rows vs the double-double oracle"}
         print(json.dumps(out), flush=True)
         return
+    if args.workload == "reorder":
+        return reorder(args)
     if args.workload != "device":
         return streamed(args)
     import torch
