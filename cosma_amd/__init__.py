@@ -53,6 +53,13 @@ def blocks(items):
     return arr, len(items)
 
 
+class GemmDesc(ctypes.Structure):
+    """One problem of a grouped GEMM (cosma_amd_gemm_desc)."""
+    _fields_ = [("transa", ctypes.c_char), ("transb", ctypes.c_char), ("m", c_i64), ("n", c_i64),
+                ("k", c_i64), ("alpha", c_dbl), ("a", c_vp), ("lda", c_i64), ("b", c_vp), ("ldb", c_i64),
+                ("beta", c_dbl), ("c", c_vp), ("ldc", c_i64)]
+
+
 class OzakiInfo(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int), ("count", ctypes.c_int), ("bits", ctypes.c_int),
                 ("int8_products", ctypes.c_int64), ("error_bound", ctypes.c_double)]
@@ -81,6 +88,14 @@ SIGNATURES = {
     "cosma_amd_handle_last_mode": (ctypes.c_int, [c_vp]),
     "cosma_amd_fill_lognormal": (ctypes.c_int, [c_vp, c_i64, c_i64, c_i64, ctypes.c_uint64, c_dbl, c_vp]),
     "cosma_amd_dgemm_set_tuning": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    "cosma_amd_grouped_create": (ctypes.c_int, [c_vp, c_vp, c_i64, ctypes.c_int, ctypes.POINTER(c_vp)]),
+    "cosma_amd_grouped_run": (ctypes.c_int, [c_vp, c_vp]),
+    "cosma_amd_grouped_graph_run": (ctypes.c_int, [c_vp, c_vp]),
+    "cosma_amd_grouped_tune": (ctypes.c_int, [c_vp, c_vp]),
+    "cosma_amd_grouped_set_config": (ctypes.c_int, [c_vp, ctypes.c_int, ctypes.c_int]),
+    "cosma_amd_grouped_info": (ctypes.c_int, [c_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                              ctypes.POINTER(c_i64)]),
+    "cosma_amd_grouped_destroy": (ctypes.c_int, [c_vp]),
     "cosma_amd_host_alloc": (ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(c_vp)]),
     "cosma_amd_host_free": (ctypes.c_int, [c_vp]),
     "cosma_amd_dgemm_layout": (ctypes.c_int, [c_vp, c_char, c_char, c_i64, c_i64, c_i64, c_dbl,
@@ -194,6 +209,51 @@ class Handle:
     def block_transform(self, op, rows, cols, alpha, src, ld_src, beta, dst, ld_dst, stream=None):
         return check(lib().cosma_amd_block_transform(self.ptr, _ch(op), rows, cols, alpha, src, ld_src,
                                                      beta, dst, ld_dst, stream))
+
+
+class Grouped:
+    """A grouped-GEMM plan over device-resident problems.
+
+    problems: iterable of (transa, transb, m, n, k, alpha, a_ptr, lda, b_ptr, ldb, beta, c_ptr, ldc).
+    """
+
+    def __init__(self, handle, problems, tile_config=0):
+        probs = list(problems)
+        arr = (GemmDesc * max(1, len(probs)))()
+        for i, (ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc) in enumerate(probs):
+            arr[i] = GemmDesc(_ch(ta), _ch(tb), m, n, k, alpha, a, lda, b, ldb, beta, c, ldc)
+        p = c_vp()
+        check(lib().cosma_amd_grouped_create(handle.ptr, ctypes.cast(arr, c_vp), len(probs), tile_config,
+                                             ctypes.byref(p)))
+        self.ptr = p
+
+    def run(self, stream=None):
+        check(lib().cosma_amd_grouped_run(self.ptr, stream))
+
+    def graph_run(self, stream=None):
+        check(lib().cosma_amd_grouped_graph_run(self.ptr, stream))
+
+    def tune(self, stream=None):
+        check(lib().cosma_amd_grouped_tune(self.ptr, stream))
+
+    def set_config(self, tile_config, grid=0):
+        check(lib().cosma_amd_grouped_set_config(self.ptr, tile_config, grid))
+
+    def info(self):
+        cfg, grid, tiles = ctypes.c_int(), ctypes.c_int(), c_i64()
+        check(lib().cosma_amd_grouped_info(self.ptr, ctypes.byref(cfg), ctypes.byref(grid), ctypes.byref(tiles)))
+        return {"tile_config": cfg.value, "grid": grid.value, "tiles": tiles.value}
+
+    def close(self):
+        if self.ptr:
+            lib().cosma_amd_grouped_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def fill_uniform(ptr, rows, cols, ld, seed, offset=0, stream=None):

@@ -231,6 +231,52 @@ int cosma_amd_dgemm_layout(cosma_amd_handle_t h, char transa, char transb, int64
                         n_c_blocks);
 }
 
+int cosma_amd_grouped_create(cosma_amd_handle_t h, const cosma_amd_gemm_desc *problems, int64_t count,
+                             int tile_config, cosma_amd_grouped_t *plan) {
+    clear_error();
+    CA_CHECK(h != nullptr, "grouped_create: null handle");
+    CA_CHECK(plan != nullptr, "grouped_create: null plan pointer");
+    CA_HIP(hipSetDevice(h->device));
+    return grouped_create(h, problems, count, tile_config, plan);
+}
+
+int cosma_amd_grouped_run(cosma_amd_grouped_t plan, void *stream) {
+    clear_error();
+    CA_CHECK(plan != nullptr, "grouped_run: null plan");
+    return grouped_run(plan, static_cast<hipStream_t>(stream));
+}
+
+int cosma_amd_grouped_graph_run(cosma_amd_grouped_t plan, void *stream) {
+    clear_error();
+    CA_CHECK(plan != nullptr, "grouped_graph_run: null plan");
+    return grouped_graph_run(plan, static_cast<hipStream_t>(stream));
+}
+
+int cosma_amd_grouped_tune(cosma_amd_grouped_t plan, void *stream) {
+    clear_error();
+    CA_CHECK(plan != nullptr, "grouped_tune: null plan");
+    return grouped_tune(plan, static_cast<hipStream_t>(stream));
+}
+
+int cosma_amd_grouped_set_config(cosma_amd_grouped_t plan, int tile_config, int grid) {
+    clear_error();
+    CA_CHECK(plan != nullptr, "grouped_set_config: null plan");
+    return grouped_set_config(plan, tile_config, grid);
+}
+
+int cosma_amd_grouped_info(cosma_amd_grouped_t plan, int *tile_config, int *grid, int64_t *tiles) {
+    clear_error();
+    CA_CHECK(plan != nullptr, "grouped_info: null plan");
+    grouped_info(plan, tile_config, grid, tiles);
+    return COSMA_AMD_OK;
+}
+
+int cosma_amd_grouped_destroy(cosma_amd_grouped_t plan) {
+    clear_error();
+    grouped_destroy(plan);
+    return COSMA_AMD_OK;
+}
+
 int cosma_amd_host_alloc(size_t bytes, void **ptr) {
     clear_error();
     CA_CHECK(ptr != nullptr, "host_alloc: null output pointer");

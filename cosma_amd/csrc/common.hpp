@@ -92,6 +92,35 @@ struct cosma_amd_handle_s {
 };
 
 namespace cosma_amd {
+// One problem of a grouped GEMM as the device sees it.
+struct GroupedProblem {
+    int64_t m, n, k;
+    double alpha, beta;
+    const double *a;
+    int64_t lda;
+    const double *b;
+    int64_t ldb;
+    double *c;
+    int64_t ldc;
+    int64_t tile_start;   // index of the problem's first tile in the launch
+    int flags;            // bit 0: A transposed, bit 1: B transposed, bit 2: 16-byte operand loads
+    int vec_c;            // 16-byte C accesses
+};
+int grouped_tile_shape(int cfg, int *bm, int *bn);
+int grouped_problem_flags(char ta, char tb, int64_t m, int64_t n, const double *a, int64_t lda, const double *b,
+                          int64_t ldb, const double *c, int64_t ldc, int *flags, int *vec_c);
+int launch_grouped(int cfg, const GroupedProblem *probs, int nprob, int64_t total_tiles, int grid, hipStream_t s);
+int grouped_create(cosma_amd_handle_t h, const cosma_amd_gemm_desc *d, int64_t count, int tile_config,
+                   cosma_amd_grouped_t *out);
+int grouped_run(cosma_amd_grouped_t p, hipStream_t s);
+int grouped_graph_run(cosma_amd_grouped_t p, hipStream_t s);
+int grouped_tune(cosma_amd_grouped_t p, hipStream_t s);
+int grouped_set_config(cosma_amd_grouped_t p, int tile_config, int grid);
+void grouped_info(cosma_amd_grouped_t p, int *tile_config, int *grid, int64_t *tiles);
+void grouped_destroy(cosma_amd_grouped_t p);
+}  // namespace cosma_amd
+
+namespace cosma_amd {
 // Kernel launchers (stream already resolved, arguments already validated).
 int launch_dgemm(char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha,
                  const double *a, int64_t lda, const double *b, int64_t ldb,

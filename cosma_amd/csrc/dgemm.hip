@@ -160,30 +160,29 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t T) {
     return (xcd < r) ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
 }
 
+template <int BM, int BN>
+struct TileSmem {
+    f64x2 a[2][KP][BM + 1];
+    f64x2 b[2][KP][BN + 1];
+};
+
+// One BM x BN tile of C at (m0, n0): the whole k loop and the epilogue,
+// executed by the 256 threads of the workgroup.  Used by the single-GEMM
+// kernel (one tile per workgroup) and by the grouped kernel (a persistent
+// loop over the tiles of many problems); the caller provides the LDS tiles
+// and must not touch them while this runs.
 template <int BM, int BN, bool TA, bool TB, bool VEC>
-__global__ __launch_bounds__(NT) void dgemm_mfma_kernel(
+__device__ __forceinline__ void gemm_tile(
     int64_t M, int64_t N, int64_t K, double alpha,
     const double *__restrict__ A, int64_t lda,
     const double *__restrict__ B, int64_t ldb,
     double beta, double *__restrict__ C, int64_t ldc,
-    int vec_c, int group_m) {
+    int vec_c, int64_t m0, int64_t n0, TileSmem<BM, BN> &sm) {
     constexpr int FM = BM / 32, FN = BN / 32;    // MFMA blocks per wave
     using IOA = TileIO<!TA, BM, VEC>;
     using IOB = TileIO<TB, BN, VEC>;
-    __shared__ f64x2 sa[2][KP][BM + 1];
-    __shared__ f64x2 sb[2][KP][BN + 1];
-
-    // ---- tile rasterisation (XCD-aware, grouped along m)
-    const int64_t tiles_m = (M + BM - 1) / BM;
-    const int64_t tiles_n = (N + BN - 1) / BN;
-    const int64_t bid = xcd_remap(blockIdx.x, tiles_m * tiles_n);
-    const int64_t per_group = (int64_t)group_m * tiles_n;
-    const int64_t grp = bid / per_group;
-    const int64_t first_m = grp * group_m;
-    const int64_t gsize = min((int64_t)group_m, tiles_m - first_m);
-    const int64_t in_group = bid - grp * per_group;
-    const int64_t m0 = (first_m + in_group % gsize) * BM;
-    const int64_t n0 = (in_group / gsize) * BN;
+    auto &sa = sm.a;
+    auto &sb = sm.b;
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -351,6 +350,28 @@ __global__ __launch_bounds__(NT) void dgemm_mfma_kernel(
     }
 }
 
+template <int BM, int BN, bool TA, bool TB, bool VEC>
+__global__ __launch_bounds__(NT) void dgemm_mfma_kernel(
+    int64_t M, int64_t N, int64_t K, double alpha,
+    const double *__restrict__ A, int64_t lda,
+    const double *__restrict__ B, int64_t ldb,
+    double beta, double *__restrict__ C, int64_t ldc,
+    int vec_c, int group_m) {
+    __shared__ TileSmem<BM, BN> sm;
+    // ---- tile rasterisation (XCD-aware, grouped along m)
+    const int64_t tiles_m = (M + BM - 1) / BM;
+    const int64_t tiles_n = (N + BN - 1) / BN;
+    const int64_t bid = xcd_remap(blockIdx.x, tiles_m * tiles_n);
+    const int64_t per_group = (int64_t)group_m * tiles_n;
+    const int64_t grp = bid / per_group;
+    const int64_t first_m = grp * group_m;
+    const int64_t gsize = min((int64_t)group_m, tiles_m - first_m);
+    const int64_t in_group = bid - grp * per_group;
+    const int64_t m0 = (first_m + in_group % gsize) * BM;
+    const int64_t n0 = (in_group / gsize) * BN;
+    gemm_tile<BM, BN, TA, TB, VEC>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, vec_c, m0, n0, sm);
+}
+
 // C <- beta * C; beta == 0 writes zeros without reading C.
 __global__ void scale_kernel(int64_t M, int64_t N, double beta, double *__restrict__ C, int64_t ldc) {
     const int64_t total = M * N;
@@ -399,6 +420,50 @@ int launch_cfg(char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha,
     return COSMA_AMD_OK;
 }
 
+// ------------------------------------------------------------ grouped GEMM
+// One persistent launch over the tiles of many independent problems.  Tiles
+// are numbered problem by problem (problems in the order of the descriptor
+// array, which the host sorts by decreasing cost) and column-major inside a
+// problem; workgroup w takes tiles w, w + G, w + 2G, ...  Each tile runs the
+// same gemm_tile as the single-GEMM kernel, so a problem's result is
+// bit-identical to a cosma_amd_dgemm call on it.
+template <int BM, int BN>
+__global__ __launch_bounds__(NT) void dgemm_grouped_kernel(const GroupedProblem *__restrict__ probs, int nprob,
+                                                           int64_t total_tiles) {
+    __shared__ TileSmem<BM, BN> sm;
+    for (int64_t t = blockIdx.x; t < total_tiles; t += gridDim.x) {
+        // problem owning tile t: last p with tile_start <= t
+        int lo = 0, hi = nprob - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (probs[mid].tile_start <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const GroupedProblem &q = probs[lo];
+        const int64_t local = t - q.tile_start;
+        const int64_t tiles_m = (q.m + BM - 1) / BM;
+        const int64_t m0 = (local % tiles_m) * BM, n0 = (local / tiles_m) * BN;
+        const bool scale_only = (q.k == 0 || q.alpha == 0.0);
+        const int64_t K = scale_only ? 0 : q.k;
+        const double alpha = scale_only ? 0.0 : q.alpha;
+        __syncthreads();   // the previous tile's last LDS reads are done
+#define CA_TILE(X, Y, V)                                                                              \
+    gemm_tile<BM, BN, X, Y, V>(q.m, q.n, K, alpha, q.a, q.lda, q.b, q.ldb, q.beta, q.c, q.ldc, q.vec_c, m0, n0, sm)
+        const int f = q.flags;   // bit 0: A transposed, bit 1: B transposed, bit 2: 16-byte loads
+        switch (f) {
+        case 0: CA_TILE(false, false, false); break;
+        case 1: CA_TILE(true, false, false); break;
+        case 2: CA_TILE(false, true, false); break;
+        case 3: CA_TILE(true, true, false); break;
+        case 4: CA_TILE(false, false, true); break;
+        case 5: CA_TILE(true, false, true); break;
+        case 6: CA_TILE(false, true, true); break;
+        default: CA_TILE(true, true, true); break;
+        }
+#undef CA_TILE
+    }
+}
+
 }  // namespace
 
 int launch_scale(int64_t m, int64_t n, double beta, double *c, int64_t ldc, hipStream_t s) {
@@ -445,6 +510,42 @@ int launch_dgemm(char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha
 void set_dgemm_tuning(int cfg, int group_m) {
     g_cfg_override = cfg;
     g_group_override = group_m;
+}
+
+int grouped_tile_shape(int cfg, int *bm, int *bn) {
+    switch (cfg) {
+    case 1: *bm = 64; *bn = 64; return COSMA_AMD_OK;
+    case 2: *bm = 128; *bn = 64; return COSMA_AMD_OK;
+    case 3: *bm = 128; *bn = 128; return COSMA_AMD_OK;
+    default:
+        set_error("grouped dgemm: unknown tile configuration %d", cfg);
+        return COSMA_AMD_ERR_INVALID_ARG;
+    }
+}
+
+int grouped_problem_flags(char ta, char tb, int64_t m, int64_t n, const double *a, int64_t lda, const double *b,
+                          int64_t ldb, const double *c, int64_t ldc, int *flags, int *vec_c) {
+    const bool TA = ta == 'T', TB = tb == 'T';
+    const bool vec = aligned16(a) && (lda % 2 == 0) && aligned16(b) && (ldb % 2 == 0) && (TA || m % 2 == 0) &&
+                     (!TB || n % 2 == 0);
+    *flags = (TA ? 1 : 0) | (TB ? 2 : 0) | (vec ? 4 : 0);
+    *vec_c = aligned16(c) && (ldc % 2 == 0);
+    return COSMA_AMD_OK;
+}
+
+int launch_grouped(int cfg, const GroupedProblem *probs, int nprob, int64_t total_tiles, int grid, hipStream_t s) {
+    if (nprob == 0 || total_tiles == 0) return COSMA_AMD_OK;
+    const dim3 g((unsigned)grid), b(NT);
+    switch (cfg) {
+    case 1: hipLaunchKernelGGL((dgemm_grouped_kernel<64, 64>), g, b, 0, s, probs, nprob, total_tiles); break;
+    case 2: hipLaunchKernelGGL((dgemm_grouped_kernel<128, 64>), g, b, 0, s, probs, nprob, total_tiles); break;
+    case 3: hipLaunchKernelGGL((dgemm_grouped_kernel<128, 128>), g, b, 0, s, probs, nprob, total_tiles); break;
+    default:
+        set_error("grouped dgemm: unknown tile configuration %d", cfg);
+        return COSMA_AMD_ERR_INVALID_ARG;
+    }
+    CA_HIP(hipGetLastError());
+    return COSMA_AMD_OK;
 }
 
 }  // namespace cosma_amd

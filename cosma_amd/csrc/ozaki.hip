@@ -15,7 +15,7 @@
 //     modulo N pairwise-coprime moduli m_l <= 256 (each residue fits int8).
 //     One INT8 GEMM per modulus gives A'B' mod m_l exactly (INT32
 //     accumulation); the Chinese-remainder recombination of the N residues
-//     recovers the exact integer A'B' (|A'B'| < M / 4, M = prod m_l), which is
+//     recovers the exact integer A'B' (|A'B'| < M / 2, M = prod m_l), which is
 //     scaled back by 2^(E_i + F_j - 2P).  The only approximation is the
 //     rounding of the inputs to P bits (relative to their row / column max).
 //   * slices: the scaled entries are written as sums of s signed 7-bit digits
@@ -247,7 +247,9 @@ __global__ void crt_epilogue(CrtConst cc, const int8_t *__restrict__ res, int64_
         for (int e = 0; e < 4; ++e) {
             const int64_t i = i0 + e;
             if (i >= M) break;
-            // q = round(S / M); |A'B'| < M / 4 keeps the rounding away from ties
+            // q = round(S / M); |A'B'| < M / 2 by the choice of P, and sd * inv_M
+            // is within ~2^-40 of S / M, so the rounding cannot fall on the
+            // wrong side of a tie
             const double sd = ldexp(s[e][2], 2 * LIMB) + ldexp(s[e][1], LIMB) + s[e][0];
             const int64_t q = (int64_t)rint(sd * cc.inv_M);
             int64_t t0 = (int64_t)s[e][0] - q * cc.mlimb[0], t1 = (int64_t)s[e][1] - q * cc.mlimb[1];
@@ -300,10 +302,14 @@ double log2_moduli(int n) {
     return s;
 }
 
-// Largest P with 4 * k * 2^(2P) < M (so that |A'B'| < M / 4).
+// Largest P with k * 2^(2P) < M / 2 (with a relative margin of ~1e-6): the
+// integer inputs satisfy |a'| <= 2^P, so |A'B'| <= k * 2^(2P) stays inside
+// the symmetric residue range (-M/2, M/2).  The recombination estimates the
+// CRT quotient from an FP64 value of S / M that is accurate to ~2^-40, far
+// inside that margin.
 int moduli_bits(int n, int64_t k) {
     const double lm = log2_moduli(n);
-    return (int)std::floor((lm - 2.0 - std::log2((double)std::max<int64_t>(k, 1))) / 2.0 - 1e-9);
+    return (int)std::floor((lm - 1.0 - std::log2((double)std::max<int64_t>(k, 1)) - 1e-6) / 2.0);
 }
 
 static void crt_constants(int n, CrtConst &cc) {
@@ -350,7 +356,9 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
             set_error("ozaki: %d moduli requested (2..%d supported)", count, MAX_MODULI);
             return COSMA_AMD_ERR_INVALID_ARG;
         }
-        P = std::min(moduli_bits(count, k), 52);
+        // residues stay exact while a' / m_l < 2^52 (one correction step in
+        // the split) and a' fits int64: P <= 60
+        P = std::min(moduli_bits(count, k), 60);
         if (P < 1) {
             set_error("ozaki: %d moduli leave no precision for k = %lld", count, (long long)k);
             return COSMA_AMD_ERR_UNSUPPORTED;

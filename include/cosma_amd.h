@@ -193,6 +193,53 @@ int cosma_amd_block_transform(cosma_amd_handle_t handle, char op,
                               const double *src, int64_t ld_src, double beta,
                               double *dst, int64_t ld_dst, void *stream);
 
+/* ------------------------------------------------------- grouped GEMM */
+/* One problem of a grouped GEMM: C <- alpha * op(A) * op(B) + beta * C. */
+typedef struct {
+    char transa, transb;
+    int64_t m, n, k;
+    double alpha;
+    const double *a;
+    int64_t lda;
+    const double *b;
+    int64_t ldb;
+    double beta;
+    double *c;
+    int64_t ldc;
+} cosma_amd_gemm_desc;
+
+typedef struct cosma_amd_grouped_s *cosma_amd_grouped_t;
+
+/*
+ * Plans a grouped GEMM over `count` independent device-resident problems
+ * (same semantics per problem as cosma_amd_dgemm; problems with m or n == 0
+ * are dropped).  The descriptors are copied to the device; the operand
+ * pointers must stay valid while the plan is used.  tile_config: 0 = the
+ * tuned choice for these shapes if one exists in the process, else 64x64;
+ * 1 = 64x64, 2 = 128x64, 3 = 128x128 C tiles.
+ * cosma_amd_grouped_run: one persistent kernel launch on `stream` (no
+ * allocation, memset or synchronisation: capturable).
+ * cosma_amd_grouped_graph_run: captures the launch into a hipGraph on first
+ * use and launches the graph on `stream`.
+ * cosma_amd_grouped_tune: times every tile shape with several persistent
+ * grid sizes on the plan's own problems (overwriting their C), keeps the
+ * fastest and records it for later plans with the same shapes.
+ * Each problem's result is bit-identical to cosma_amd_dgemm on it.
+ * No counterpart in the reference (its local multiplies are one GEMM each,
+ * src/cosma/local_multiply.cpp:299-381); on one rank the sequential steps of
+ * src/cosma/multiply.cpp:460-588 are one client.
+ */
+int cosma_amd_grouped_create(cosma_amd_handle_t handle, const cosma_amd_gemm_desc *problems,
+                             int64_t count, int tile_config, cosma_amd_grouped_t *plan);
+int cosma_amd_grouped_run(cosma_amd_grouped_t plan, void *stream);
+int cosma_amd_grouped_graph_run(cosma_amd_grouped_t plan, void *stream);
+int cosma_amd_grouped_tune(cosma_amd_grouped_t plan, void *stream);
+/* Forces a tile configuration (1..3) and grid (<= 0: default) for the plan. */
+int cosma_amd_grouped_set_config(cosma_amd_grouped_t plan, int tile_config, int grid);
+/* Current tile configuration, grid and number of tiles of the plan. */
+int cosma_amd_grouped_info(cosma_amd_grouped_t plan, int *tile_config, int *grid, int64_t *tiles);
+int cosma_amd_grouped_destroy(cosma_amd_grouped_t plan);
+
 /* ------------------------------------------------------ pinned host memory */
 /*
  * Allocates `bytes` of page-locked host memory (hipHostMalloc) that the
