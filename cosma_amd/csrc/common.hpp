@@ -82,6 +82,7 @@ struct cosma_amd_handle_s {
     cosma_amd::DevBuf dA, dB, dC;       // resident operand copies / ring slots
     cosma_amd::DevBuf dX;               // product before the layout epilogue
     cosma_amd::DevBuf ozaki_ws;         // split-precision workspace
+    cosma_amd::DevBuf ozaki_aux;        // split-precision row statistics and exponents
     std::vector<hipEvent_t> events;     // event pool
     size_t events_used = 0;
     std::vector<cosma_amd::StagingSlot> h2d_slots, d2h_slots;

@@ -26,8 +26,10 @@
 // recombination, and the alpha * AB + beta * C epilogue.
 // The number of moduli (or slices) is chosen from the requested tolerance; the
 // a-priori bound and the count used are returned to the caller.
+#include <algorithm>
 #include <climits>
 #include <cmath>
+#include <vector>
 
 #include "i8gemm.hpp"
 
@@ -35,17 +37,25 @@ namespace cosma_amd {
 
 // pairwise-coprime moduli, largest first (256 = 2^8; 255 = 3*5*17; 253 = 11*23;
 // 247 = 13*19; 217 = 7*31; the rest are primes)
-static const int kModuli[16] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193};
-constexpr int MAX_MODULI = 16;
-constexpr int LIMB = 42;   // CRT limb width in bits (3 limbs cover M < 2^126)
+static const int kModuli[20] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227,
+                                223, 217, 211, 199, 197, 193, 191, 181, 179, 173};
+constexpr int MAX_MODULI = 20;
+// CRT limbs: 4 limbs of 41 bits cover M < 2^164 (20 moduli: M ~ 2^155.4), and a
+// limb sum of 20 terms |c_l| * w < 20 * 2^7 * 2^41 < 2^53 stays exact in FP64.
+constexpr int LIMB = 41;
+constexpr int NLIMB = 4;
+// Largest P (bits of the integer inputs) for which the residues stay exact:
+// a' / m_l < 2^51 for every modulus (>= 173) keeps the split's FP64 quotient
+// within one of the true one, so a single correction step is exact.
+constexpr int MAX_P = 58;
 
 struct CrtConst {
     int n;
     int m[MAX_MODULI];
     double inv_m[MAX_MODULI];
-    double w[MAX_MODULI][3];    // CRT weights W_l in 42-bit limbs (exact in FP64)
-    int64_t mlimb[3];           // M in 42-bit limbs
-    double inv_M;               // 1 / M
+    double w[MAX_MODULI][NLIMB];   // CRT weights W_l in 41-bit limbs (exact in FP64)
+    int64_t mlimb[NLIMB];          // M in 41-bit limbs
+    double inv_M;                  // 1 / M
 };
 
 namespace {
@@ -59,46 +69,53 @@ __global__ void clear_u64(unsigned long long *p, int64_t n) {
         p[i] = 0ull;
 }
 
-// rows of op(X) are strided in memory (X 'N' for A, X 'T' for B): thread per
-// row, 64 columns per block-row of the grid.
-__global__ void rowmax_strided(const double *__restrict__ x, int64_t ld, int64_t R, int64_t K,
-                               unsigned long long *maxbits) {
-    const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (r >= R) return;
-    const int64_t k0 = (int64_t)blockIdx.y * 64, k1 = min(K, k0 + 64);
-    unsigned long long best = 0;
-    for (int64_t k = k0; k < k1; ++k) {
-        const unsigned long long b = __double_as_longlong(fabs(x[r + k * ld]));
-        best = b > best ? b : best;
+// Statistics of op(X) seen as R vectors of length K (the rows of op(A), the
+// columns of op(B)): the max |x| of every vector (as bit patterns, for the
+// exponents) and, for every k, the sum of squares across the vectors (for the
+// error estimate).  One 64 x 64 tile per workgroup through LDS, loaded along
+// the contiguous dimension; one atomic per vector and per k per tile.
+// CONTIG: element (r, k) at x[k + r * ld]; otherwise at x[r + k * ld].
+template <bool CONTIG>
+__global__ void stats_kernel(const double *__restrict__ x, int64_t ld, int64_t R, int64_t K,
+                             unsigned long long *maxbits, double *ksq) {
+    __shared__ double tile[64][65];
+    const int64_t r0 = (int64_t)blockIdx.x * 64, k0 = (int64_t)blockIdx.y * 64;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int idx = threadIdx.x + 256 * q;
+        const int fast = idx & 63, slow = idx >> 6;
+        const int rr = CONTIG ? slow : fast, kk = CONTIG ? fast : slow;
+        const int64_t r = r0 + rr, k = k0 + kk;
+        double v = 0.0;
+        if (r < R && k < K) v = CONTIG ? x[k + r * ld] : x[r + k * ld];
+        tile[rr][kk] = v;
     }
-    atomicMax(&maxbits[r], best);
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int rr = threadIdx.x;
+        unsigned long long best = 0;
+        for (int kk = 0; kk < 64; ++kk) {
+            const unsigned long long b = __double_as_longlong(fabs(tile[rr][kk]));
+            best = b > best ? b : best;
+        }
+        if (r0 + rr < R) atomicMax(&maxbits[r0 + rr], best);
+    } else if (threadIdx.x < 128) {
+        const int kk = threadIdx.x - 64;
+        double ss = 0.0;
+        for (int rr = 0; rr < 64; ++rr) ss = fma(tile[rr][kk], tile[rr][kk], ss);
+        if (k0 + kk < K) atomicAdd(&ksq[k0 + kk], ss);
+    }
 }
 
-// rows of op(X) are contiguous (X 'T' for A, 'N' for B): block per (row, chunk)
-__global__ void rowmax_contig(const double *__restrict__ x, int64_t ld, int64_t R, int64_t K,
-                              unsigned long long *maxbits) {
-    const int64_t r = blockIdx.x;
-    const int64_t k0 = (int64_t)blockIdx.y * 8192, k1 = min(K, k0 + 8192);
-    unsigned long long best = 0;
-    for (int64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
-        const unsigned long long b = __double_as_longlong(fabs(x[k + r * ld]));
-        best = b > best ? b : best;
-    }
-    __shared__ unsigned long long red[256];
-    red[threadIdx.x] = best;
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s && red[threadIdx.x + s] > red[threadIdx.x]) red[threadIdx.x] = red[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) atomicMax(&maxbits[r], red[0]);
-}
+// Exponent given to an all-zero row or column: its entries and products are
+// zero whatever the scaling, and it drops out of the error estimate.
+constexpr int ZERO_EXPONENT = -30000;
 
 // Row exponent from the max: max < 2^E.  For slices the leading digit must
 // stay within [-127, 127]: rows whose scaled max would round to 128 take E + 1.
 __device__ __forceinline__ int row_exponent(unsigned long long bits, int slices) {
     const double mx = __longlong_as_double((long long)bits);
-    if (mx == 0.0) return 0;
+    if (mx == 0.0) return ZERO_EXPONENT;
     int e;
     const double f = frexp(mx, &e);   // mx = f * 2^e, f in [0.5, 1)
     if (slices && f * 128.0 >= 127.5) ++e;
@@ -227,19 +244,20 @@ __global__ void crt_epilogue(CrtConst cc, const int8_t *__restrict__ res, int64_
         const int64_t j = idx / quads, i0 = (idx - j * quads) * 4;
         if (i0 >= M) continue;
         const int8_t *rp = res + i0 + j * ldr;
-        // S = sum_l c_l W_l limb by limb: |c_l| <= 128 and limbs < 2^42, so
-        // every partial sum of at most 16 terms stays below 2^53 (exact FMAs)
-        double s[4][3];
+        // S = sum_l c_l W_l limb by limb: |c_l| <= 128 and limbs < 2^41, so
+        // every partial sum of at most 20 terms stays below 2^53 (exact FMAs)
+        double s[4][NLIMB];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) s[e][0] = s[e][1] = s[e][2] = 0.0;
+        for (int e = 0; e < 4; ++e)
+#pragma unroll
+            for (int q = 0; q < NLIMB; ++q) s[e][q] = 0.0;
         for (int l = 0; l < cc.n; ++l) {
             const uint32_t word = *reinterpret_cast<const uint32_t *>(rp + (int64_t)l * res_stride);
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const double c = (double)(int8_t)(word >> (8 * e));
-                s[e][0] = fma(c, cc.w[l][0], s[e][0]);
-                s[e][1] = fma(c, cc.w[l][1], s[e][1]);
-                s[e][2] = fma(c, cc.w[l][2], s[e][2]);
+#pragma unroll
+                for (int q = 0; q < NLIMB; ++q) s[e][q] = fma(c, cc.w[l][q], s[e][q]);
             }
         }
         const double fj = (double)F[j];
@@ -250,22 +268,28 @@ __global__ void crt_epilogue(CrtConst cc, const int8_t *__restrict__ res, int64_
             // q = round(S / M); |A'B'| < M / 2 by the choice of P, and sd * inv_M
             // is within ~2^-40 of S / M, so the rounding cannot fall on the
             // wrong side of a tie
-            const double sd = ldexp(s[e][2], 2 * LIMB) + ldexp(s[e][1], LIMB) + s[e][0];
+            const double sd = ldexp(s[e][3], 3 * LIMB) + ldexp(s[e][2], 2 * LIMB) + ldexp(s[e][1], LIMB) + s[e][0];
             const int64_t q = (int64_t)rint(sd * cc.inv_M);
-            int64_t t0 = (int64_t)s[e][0] - q * cc.mlimb[0], t1 = (int64_t)s[e][1] - q * cc.mlimb[1];
-            int64_t t2 = (int64_t)s[e][2] - q * cc.mlimb[2];
-            int64_t carry = t0 >> LIMB;              // arithmetic shift: floor division
-            t0 -= carry << LIMB;
-            t1 += carry;
-            carry = t1 >> LIMB;
-            t1 -= carry << LIMB;
-            t2 += carry;
-            // value = t2 * 2^84 + t1 * 2^42 + t0, limbs exact in double
-            const double hi = ldexp((double)t2, 2 * LIMB), mid = ldexp((double)t1, LIMB);
-            const double sm = hi + mid;
-            const double bb = sm - hi;
-            const double err = (hi - (sm - bb)) + (mid - bb);   // TwoSum error of hi + mid
-            const double v = sm + (err + (double)t0);
+            int64_t t[NLIMB];
+#pragma unroll
+            for (int x = 0; x < NLIMB; ++x) t[x] = (int64_t)s[e][x] - q * cc.mlimb[x];
+#pragma unroll
+            for (int x = 0; x < NLIMB - 1; ++x) {    // normalise: limbs 0..2 in [0, 2^41)
+                const int64_t carry = t[x] >> LIMB;  // arithmetic shift: floor division
+                t[x] -= carry << LIMB;
+                t[x + 1] += carry;
+            }
+            // value = t3 2^123 + t2 2^82 + t1 2^41 + t0, each term exact in FP64;
+            // compensated sum of the three largest, then the low limb
+            const double h3 = ldexp((double)t[3], 3 * LIMB), h2 = ldexp((double)t[2], 2 * LIMB);
+            const double h1 = ldexp((double)t[1], LIMB);
+            double sm = h3 + h2;
+            double bb = sm - h3;
+            double err = (h3 - (sm - bb)) + (h2 - bb);
+            const double sm2 = sm + h1;
+            bb = sm2 - sm;
+            err += (sm - (sm2 - bb)) + (h1 - bb);
+            const double v = sm2 + (err + (double)t[0]);
             const double val = alpha * ldexp(v, E[i] + (int)fj - 2 * P);
             double *cp = C + i + j * ldc;
             *cp = (beta == 0.0) ? val : fma(beta, *cp, val);
@@ -312,31 +336,96 @@ int moduli_bits(int n, int64_t k) {
     return (int)std::floor((lm - 1.0 - std::log2((double)std::max<int64_t>(k, 1)) - 1e-6) / 2.0);
 }
 
+namespace {
+// Little-endian 256-bit unsigned integer, enough for M = prod of up to 20
+// moduli (< 2^160) and the CRT weights W_l < M.
+struct U256 {
+    uint64_t w[4] = {0, 0, 0, 0};
+};
+
+void mul_small(U256 &x, uint32_t v) {
+    unsigned __int128 carry = 0;
+    for (int i = 0; i < 4; ++i) {
+        const unsigned __int128 t = (unsigned __int128)x.w[i] * v + carry;
+        x.w[i] = (uint64_t)t;
+        carry = t >> 64;
+    }
+}
+
+uint32_t div_small(U256 &x, uint32_t v) {   // x /= v, returns x % v
+    unsigned __int128 rem = 0;
+    for (int i = 3; i >= 0; --i) {
+        const unsigned __int128 cur = (rem << 64) | x.w[i];
+        x.w[i] = (uint64_t)(cur / v);
+        rem = cur % v;
+    }
+    return (uint32_t)rem;
+}
+
+uint64_t bit_field(const U256 &x, int lo, int n) {   // bits [lo, lo + n), n <= 63
+    uint64_t out = 0;
+    for (int b = 0; b < n; ++b) {
+        const int bit = lo + b;
+        if (bit < 256 && ((x.w[bit / 64] >> (bit % 64)) & 1)) out |= (uint64_t)1 << b;
+    }
+    return out;
+}
+
+double to_double(const U256 &x) {
+    double d = 0.0;
+    for (int i = 3; i >= 0; --i) d = d * 18446744073709551616.0 + (double)x.w[i];
+    return d;
+}
+}  // namespace
+
 static void crt_constants(int n, CrtConst &cc) {
-    typedef unsigned __int128 u128;
     cc.n = n;
-    u128 M = 1;
-    for (int l = 0; l < n; ++l) M *= (u128)kModuli[l];
-    const u128 mask = ((u128)1 << LIMB) - 1;
+    U256 M;
+    M.w[0] = 1;
+    for (int l = 0; l < n; ++l) mul_small(M, (uint32_t)kModuli[l]);
     for (int l = 0; l < n; ++l) {
         const int m = kModuli[l];
         cc.m[l] = m;
         cc.inv_m[l] = 1.0 / m;
-        const u128 Ml = M / (u128)m;
-        const int r = (int)(Ml % (u128)m);
+        U256 Ml = M;
+        div_small(Ml, (uint32_t)m);                  // M / m_l, exact
+        U256 tmp = Ml;
+        const int r = (int)div_small(tmp, (uint32_t)m);   // (M / m_l) mod m_l
         int inv = 1;
         for (int y = 1; y < m; ++y)
             if ((r * y) % m == 1) { inv = y; break; }
-        const u128 W = (Ml * (u128)inv) % M;
-        cc.w[l][0] = (double)(int64_t)(W & mask);
-        cc.w[l][1] = (double)(int64_t)((W >> LIMB) & mask);
-        cc.w[l][2] = (double)(int64_t)(W >> (2 * LIMB));
+        U256 W = Ml;
+        mul_small(W, (uint32_t)inv);                 // W_l = (M / m_l) * inv < M
+        for (int q = 0; q < NLIMB; ++q) cc.w[l][q] = (double)bit_field(W, q * LIMB, LIMB);
     }
-    cc.mlimb[0] = (int64_t)(M & mask);
-    cc.mlimb[1] = (int64_t)((M >> LIMB) & mask);
-    cc.mlimb[2] = (int64_t)(M >> (2 * LIMB));
-    cc.inv_M = 1.0 / (double)M;
+    for (int q = 0; q < NLIMB; ++q) cc.mlimb[q] = (int64_t)bit_field(M, q * LIMB, LIMB);
+    cc.inv_M = 1.0 / to_double(M);
 }
+
+namespace {
+// gamma such that the relative Frobenius error of the product of the P-bit
+// roundings is about gamma * 2^-P (see ozaki_dgemm); +inf if it cannot be
+// formed (overflowed sums), 0 for a zero product.
+double error_gamma(const std::vector<int> &E, const std::vector<int> &F, const std::vector<double> &ksqA,
+                   const std::vector<double> &ksqB) {
+    int ea = INT_MIN, fb = INT_MIN;
+    for (int e : E) ea = std::max(ea, e);
+    for (int f : F) fb = std::max(fb, f);
+    if (ea == INT_MIN || fb == INT_MIN || ea == ZERO_EXPONENT || fb == ZERO_EXPONENT) return 0.0;
+    double SA = 0.0, SB = 0.0, NA = 0.0, NB = 0.0, D = 0.0;
+    for (int e : E) SA += std::ldexp(1.0, 2 * (e - ea));
+    for (int f : F) SB += std::ldexp(1.0, 2 * (f - fb));
+    for (size_t l = 0; l < ksqA.size(); ++l) {
+        if (!std::isfinite(ksqA[l]) || !std::isfinite(ksqB[l])) return INFINITY;
+        const double a = std::ldexp(ksqA[l], -2 * ea), b = std::ldexp(ksqB[l], -2 * fb);
+        NA += a;
+        NB += b;
+        D += a * b;
+    }
+    if (!(D > 0.0)) return INFINITY;
+    return std::sqrt((SA * NB + NA * SB) / (12.0 * D));
+}
+}  // namespace
 
 int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha,
                 const double *a, int64_t lda, const double *b, int64_t ldb, double beta, double *c,
@@ -345,30 +434,105 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
     if (m == 0 || n == 0) return COSMA_AMD_OK;
     if (k == 0 || alpha == 0.0) return launch_scale(m, n, beta, c, ldc, s);
     if (tol <= 0.0) tol = 1e-14;
-    const double want_bits = std::log2(2.0 / tol);          // error model: 2 * 2^-P
+    const int slices = method == COSMA_AMD_OZAKI_SLICES;
+    if (!slices && count > 0 && (count < 2 || count > MAX_MODULI)) {
+        set_error("ozaki: %d moduli requested (2..%d supported)", count, MAX_MODULI);
+        return COSMA_AMD_ERR_INVALID_ARG;
+    }
+    if (slices && count > I8_MAX_PAIRS) {
+        set_error("ozaki: %d slices requested (1..%d supported)", count, I8_MAX_PAIRS);
+        return COSMA_AMD_ERR_INVALID_ARG;
+    }
+    if (!slices && (double)k * 16384.0 >= 2147483648.0) {
+        set_error("ozaki moduli: k = %lld too large for exact INT32 accumulation", (long long)k);
+        return COSMA_AMD_ERR_UNSUPPORTED;
+    }
+    const int64_t Mp = round_up(m, 256), Np = round_up(n, 256), Kp = round_up(k, 256);
+    const int64_t ldr = round_up(m, 4);                         // residue output ld
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    auto grow = [](DevBuf &buf, size_t need) -> int {
+        if (buf.bytes >= need) return COSMA_AMD_OK;
+        if (buf.ptr) CA_HIP(hipFree(buf.ptr));
+        buf.ptr = nullptr;
+        buf.bytes = 0;
+        hipError_t e = hipMalloc(&buf.ptr, need);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            set_error("ozaki: workspace allocation of %zu bytes failed", need);
+            return COSMA_AMD_ERR_ALLOC;
+        }
+        buf.bytes = need;
+        return COSMA_AMD_OK;
+    };
+    int rc;
+
+    // ---- 1. row / column maxima, per-k sums of squares, exponents
+    // aux: maxA [Mp], maxB [Np], ksqA [Kp], ksqB [Kp], flag [1] (cleared
+    // together, 8 bytes each), then the exponents E [Mp] and F [Np]
+    const int64_t nclear = Mp + Np + 2 * Kp + 1;
+    if ((rc = grow(h->ozaki_aux, al((size_t)nclear * 8 + (size_t)(Mp + Np) * 4)))) return rc;
+    unsigned long long *maxA = static_cast<unsigned long long *>(h->ozaki_aux.ptr);
+    unsigned long long *maxB = maxA + Mp;
+    double *ksqA = reinterpret_cast<double *>(maxB + Np);
+    double *ksqB = ksqA + Kp;
+    int *flag = reinterpret_cast<int *>(ksqB + Kp);
+    int *E = reinterpret_cast<int *>(ksqB + Kp + 1);
+    int *F = E + Mp;
+    hipLaunchKernelGGL(clear_u64, dim3(grid_for(nclear)), dim3(256), 0, s, maxA, nclear);
+    const bool a_rows_contig = ta == 'T', b_rows_contig = tb == 'N';   // op rows contiguous in k
+    const dim3 ga((unsigned)((m + 63) / 64), (unsigned)((k + 63) / 64)), gb((unsigned)((n + 63) / 64), (unsigned)((k + 63) / 64));
+    if (a_rows_contig) hipLaunchKernelGGL(stats_kernel<true>, ga, dim3(256), 0, s, a, lda, m, k, maxA, ksqA);
+    else hipLaunchKernelGGL(stats_kernel<false>, ga, dim3(256), 0, s, a, lda, m, k, maxA, ksqA);
+    if (b_rows_contig) hipLaunchKernelGGL(stats_kernel<true>, gb, dim3(256), 0, s, b, ldb, n, k, maxB, ksqB);
+    else hipLaunchKernelGGL(stats_kernel<false>, gb, dim3(256), 0, s, b, ldb, n, k, maxB, ksqB);
+    hipLaunchKernelGGL(exponents_kernel, dim3(grid_for(m)), dim3(256), 0, s, maxA, m, slices, E, flag);
+    hipLaunchKernelGGL(exponents_kernel, dim3(grid_for(n)), dim3(256), 0, s, maxB, n, slices, F, flag);
+    CA_HIP(hipGetLastError());
+    int nonfinite = 0;
+    std::vector<int> hE(m), hF(n);
+    std::vector<double> hkA(k), hkB(k);
+    CA_HIP(hipMemcpyAsync(&nonfinite, flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    CA_HIP(hipMemcpyAsync(hE.data(), E, m * sizeof(int), hipMemcpyDeviceToHost, s));
+    CA_HIP(hipMemcpyAsync(hF.data(), F, n * sizeof(int), hipMemcpyDeviceToHost, s));
+    CA_HIP(hipMemcpyAsync(hkA.data(), ksqA, k * sizeof(double), hipMemcpyDeviceToHost, s));
+    CA_HIP(hipMemcpyAsync(hkB.data(), ksqB, k * sizeof(double), hipMemcpyDeviceToHost, s));
+    CA_HIP(hipStreamSynchronize(s));
+    if (nonfinite) {
+        // Inf / NaN cannot be split; the FP64 MFMA DGEMM propagates them as IEEE does.
+        if (info) info->method = -1;
+        return launch_dgemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s);
+    }
+
+    // ---- precision.  Rounding every entry to P bits below its row / column
+    // maximum leaves errors of variance 2^(2(E_i - P)) / 12 (rows of op(A))
+    // and 2^(2(F_j - P)) / 12 (columns of op(B)); with independent signs
+    //   sum_ij E[dC_ij^2] = 2^-2P / 12 (sum_i 2^2E_i ||B||^2 + ||A||^2 sum_j 2^2F_j),
+    //   sum_ij E[C_ij^2]  = sum_l ||op(A)(:, l)||^2 ||op(B)(l, :)||^2,
+    // so the relative Frobenius error is about gamma * 2^-P.  gamma ~ 0.7 for
+    // entries of similar size and grows when entries sit far below their
+    // row / column maxima.  The count is the smallest whose estimate, doubled,
+    // meets the tolerance.
+    double gamma = error_gamma(hE, hF, hkA, hkB);
+    if (!std::isfinite(gamma)) gamma = std::ldexp(1.0, 30);    // no estimate: most precise count
     int P = 0;
-    if (method == COSMA_AMD_OZAKI_MODULI) {
+    auto est = [&](int cnt) {
+        return slices ? 2.0 * gamma * 2.0 * cnt * std::ldexp(1.0, -7 * cnt)
+                      : 2.0 * gamma * std::ldexp(1.0, -std::min(moduli_bits(cnt, k), MAX_P));
+    };
+    if (!slices) {
         if (count <= 0) {
             count = 2;
-            while (count < MAX_MODULI && moduli_bits(count, k) < want_bits) ++count;
+            while (count < MAX_MODULI && !(moduli_bits(count, k) >= 1 && est(count) <= tol)) ++count;
         }
-        if (count < 2 || count > MAX_MODULI) {
-            set_error("ozaki: %d moduli requested (2..%d supported)", count, MAX_MODULI);
-            return COSMA_AMD_ERR_INVALID_ARG;
-        }
-        // residues stay exact while a' / m_l < 2^52 (one correction step in
-        // the split) and a' fits int64: P <= 60
-        P = std::min(moduli_bits(count, k), 60);
+        P = std::min(moduli_bits(count, k), MAX_P);
         if (P < 1) {
             set_error("ozaki: %d moduli leave no precision for k = %lld", count, (long long)k);
             return COSMA_AMD_ERR_UNSUPPORTED;
         }
     } else {
-        if (count <= 0) count = (int)std::ceil((want_bits + 1.0) / 7.0);
-        count = std::max(count, 1);
-        if (count > I8_MAX_PAIRS) {
-            set_error("ozaki: %d slices requested (1..%d supported)", count, I8_MAX_PAIRS);
-            return COSMA_AMD_ERR_INVALID_ARG;
+        if (count <= 0) {
+            count = 1;
+            while (count < I8_MAX_PAIRS && est(count) > tol) ++count;
         }
         // INT32 exactness of one weight: pairs * k * 127^2 < 2^31
         if ((double)count * (double)k * 16129.0 >= 2147483648.0) {
@@ -376,68 +540,15 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
             return COSMA_AMD_ERR_UNSUPPORTED;
         }
     }
-    if (method == COSMA_AMD_OZAKI_MODULI && (double)k * 16384.0 >= 2147483648.0) {
-        set_error("ozaki moduli: k = %lld too large for exact INT32 accumulation", (long long)k);
-        return COSMA_AMD_ERR_UNSUPPORTED;
-    }
 
-    // ---- workspace
-    const int64_t Mp = round_up(m, 256), Np = round_up(n, 256), Kp = round_up(k, 256);
-    const int64_t ldr = round_up(m, 4);                         // residue output ld
+    // ---- workspace: slices / residues of A and B, then the INT8 GEMM output
     const size_t a_bytes = (size_t)count * Mp * Kp, b_bytes = (size_t)count * Np * Kp;
-    const size_t out_bytes = method == COSMA_AMD_OZAKI_MODULI ? (size_t)count * ldr * n
-                                                               : (size_t)ldr * n * 8;
-    const size_t aux = (size_t)(Mp + Np + 1) * 8 + (size_t)(Mp + Np) * 4 + 256;
-    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    const size_t need = al(a_bytes) + al(b_bytes) + al(out_bytes) + al(aux);
-    if (h->ozaki_ws.bytes < need) {
-        if (h->ozaki_ws.ptr) CA_HIP(hipFree(h->ozaki_ws.ptr));
-        h->ozaki_ws.ptr = nullptr;
-        h->ozaki_ws.bytes = 0;
-        hipError_t e = hipMalloc(&h->ozaki_ws.ptr, need);
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            set_error("ozaki: workspace allocation of %zu bytes failed", need);
-            return COSMA_AMD_ERR_ALLOC;
-        }
-        h->ozaki_ws.bytes = need;
-    }
+    const size_t out_bytes = slices ? (size_t)ldr * n * 8 : (size_t)count * ldr * n;
+    if ((rc = grow(h->ozaki_ws, al(a_bytes) + al(b_bytes) + al(out_bytes)))) return rc;
     char *ws = static_cast<char *>(h->ozaki_ws.ptr);
     int8_t *As = reinterpret_cast<int8_t *>(ws);
     int8_t *Bs = reinterpret_cast<int8_t *>(ws + al(a_bytes));
     void *out = ws + al(a_bytes) + al(b_bytes);
-    char *auxp = ws + al(a_bytes) + al(b_bytes) + al(out_bytes);
-    // aux: max |A| per row [Mp], max |B| per column [Np], non-finite flag [1]
-    // (all cleared together), then the exponents E [Mp] and F [Np]
-    unsigned long long *maxA = reinterpret_cast<unsigned long long *>(auxp);
-    unsigned long long *maxB = maxA + Mp;
-    int *flag = reinterpret_cast<int *>(maxB + Np);
-    int *E = reinterpret_cast<int *>(maxB + Np + 1);
-    int *F = E + Mp;
-
-    // ---- 1. exponents
-    hipLaunchKernelGGL(clear_u64, dim3(grid_for(Mp + Np + 1)), dim3(256), 0, s, maxA, Mp + Np + 1);
-    const bool a_rows_contig = ta == 'T', b_rows_contig = tb == 'N';   // op rows contiguous in k
-    if (a_rows_contig)
-        hipLaunchKernelGGL(rowmax_contig, dim3((unsigned)m, (unsigned)((k + 8191) / 8192)), dim3(256), 0, s, a, lda, m, k, maxA);
-    else
-        hipLaunchKernelGGL(rowmax_strided, dim3((unsigned)((m + 255) / 256), (unsigned)((k + 63) / 64)), dim3(256), 0, s, a, lda, m, k, maxA);
-    if (b_rows_contig)
-        hipLaunchKernelGGL(rowmax_contig, dim3((unsigned)n, (unsigned)((k + 8191) / 8192)), dim3(256), 0, s, b, ldb, n, k, maxB);
-    else
-        hipLaunchKernelGGL(rowmax_strided, dim3((unsigned)((n + 255) / 256), (unsigned)((k + 63) / 64)), dim3(256), 0, s, b, ldb, n, k, maxB);
-    const int slices = method == COSMA_AMD_OZAKI_SLICES;
-    hipLaunchKernelGGL(exponents_kernel, dim3(grid_for(m)), dim3(256), 0, s, maxA, m, slices, E, flag);
-    hipLaunchKernelGGL(exponents_kernel, dim3(grid_for(n)), dim3(256), 0, s, maxB, n, slices, F, flag);
-    CA_HIP(hipGetLastError());
-    int nonfinite = 0;
-    CA_HIP(hipMemcpyAsync(&nonfinite, flag, sizeof(int), hipMemcpyDeviceToHost, s));
-    CA_HIP(hipStreamSynchronize(s));
-    if (nonfinite) {
-        // Inf / NaN cannot be split; the FP64 MFMA DGEMM propagates them as IEEE does.
-        if (info) info->method = -1;
-        return launch_dgemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s);
-    }
 
     // ---- 2. split
     auto split = [&](const double *x, int64_t ld, int64_t R, int64_t Rpad, bool contig, const int *ex,
@@ -462,7 +573,6 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
     pr.b = Bs; pr.ldb = Kp; pr.b_stride = Np * Kp;
     pr.out = out; pr.ldo = ldr;
     I8Jobs jobs{};
-    int rc;
     int64_t products = 0;
     if (!slices) {
         pr.out_stride = ldr * n;
@@ -506,9 +616,8 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
         info->count = count;
         info->bits = slices ? 7 * count : P;
         info->int8_products = products;
-        // a-priori estimate of the relative Frobenius error for inputs whose
-        // entries are comparable to their row / column maxima
-        info->error_bound = slices ? 2.0 * count * std::ldexp(1.0, -7 * count) : 2.0 * std::ldexp(1.0, -P);
+        // the (doubled) estimate of the relative Frobenius error, see above
+        info->error_bound = est(count);
     }
     return COSMA_AMD_OK;
 }

@@ -260,7 +260,7 @@ int upload_op_block(cosma_amd_handle_t h, const Operand &X, double *dst, int64_t
 
 void release_stream_state(cosma_amd_handle_t h) {
     hipSetDevice(h->device);
-    for (DevBuf *b : {&h->dA, &h->dB, &h->dC, &h->dX, &h->ozaki_ws}) {
+    for (DevBuf *b : {&h->dA, &h->dB, &h->dC, &h->dX, &h->ozaki_ws, &h->ozaki_aux}) {
         if (b->ptr) hipFree(b->ptr);
         b->ptr = nullptr;
         b->bytes = 0;

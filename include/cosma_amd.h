@@ -88,7 +88,7 @@ typedef struct {
     int count;              /* moduli or slices used */
     int bits;               /* bits kept per input entry relative to its row/column maximum */
     int64_t int8_products;  /* INT8 GEMMs of size m x n x k computed */
-    double error_bound;     /* a-priori relative error estimate for inputs comparable to their maxima */
+    double error_bound;     /* estimated relative Frobenius error (doubled), from the inputs' row statistics */
 } cosma_amd_ozaki_info;
 
 /*
@@ -100,13 +100,17 @@ typedef struct {
  * computed, and FP64 recombines them and applies alpha and beta (beta == 0
  * never reads C).  FP64 arithmetic runs only in the scaling, the
  * recombination and the epilogue.
- * tolerance: requested relative accuracy (<= 0 means 1e-14); count > 0 forces
- * the number of moduli (2..16) or slices (1..8) instead.  info (may be NULL)
- * receives the method, count, bits and the error estimate.
+ * tolerance: requested relative accuracy (<= 0 means 1e-14); the count of
+ * moduli or slices is then chosen from the data: the row/column maxima and
+ * sums of squares of op(A) and op(B) give an estimate of the relative
+ * Frobenius error of each precision, and the smallest count whose doubled
+ * estimate meets the tolerance is used.  count > 0 forces the number of
+ * moduli (2..20) or slices (1..8) instead.  info (may be NULL) receives the
+ * method, count, bits and the doubled error estimate.
  * Limits: moduli need k < 131072 and slices need slices * k * 127^2 < 2^31
  * (exact INT32 sums); COSMA_AMD_ERR_UNSUPPORTED otherwise.
- * Uses a workspace owned by the handle; asynchronous except for one 4-byte
- * read-back that detects non-finite inputs.
+ * Uses workspaces owned by the handle; asynchronous except for one read-back
+ * of the row statistics (non-finite flag, exponents, sums of squares).
  * No counterpart in the reference (its local multiply is a host or vendor
  * FP64 GEMM, src/cosma/local_multiply.cpp:299-381); same argument
  * conventions as cosma_amd_dgemm.

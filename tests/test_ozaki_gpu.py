@@ -20,7 +20,7 @@ from helpers import from_dev, gemm_bound, stored_shape, to_dev
 
 pytestmark = pytest.mark.gpu
 
-MODULI = [256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193]
+MODULI = [256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193, 191, 181, 179, 173]
 
 
 def op(X, t):
@@ -106,7 +106,7 @@ def test_integer_inputs_are_exact(handle, method, count, ta, tb):
     assert np.array_equal(got, exact)
 
 
-@pytest.mark.parametrize("count", [2, 5, 9, 14, 16])
+@pytest.mark.parametrize("count", [2, 5, 9, 14, 16, 18, 20])
 @pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "T")])
 def test_moduli_equals_exact_product_of_rounded_inputs(handle, count, ta, tb):
     m, n, k = 37, 45, 129
@@ -187,6 +187,32 @@ def test_lognormal_inputs_error_reported(handle):
     assert errs[2] < 1e-13
 
 
+def test_default_tolerance_holds_for_spread_exponents(handle):
+    """phi = 4 inputs (entries spread over ~2^+-17 below their row maxima): the
+    count chosen from the data must still meet the default 1e-14, and it must
+    be larger than for uniform inputs of the same shape."""
+    import torch
+    import cosma_amd as ca
+    m = n = 192
+    k = 768
+    A = torch.empty((k, m), dtype=torch.float64, device="cuda")
+    B = torch.empty((n, k), dtype=torch.float64, device="cuda")
+    ca.fill_lognormal(A.data_ptr(), m, k, m, 7, 4.0)
+    ca.fill_lognormal(B.data_ptr(), k, n, k, 8, 4.0)
+    torch.cuda.synchronize()
+    Ah, Bh = from_dev(A, m), from_dev(B, k)
+    ref = np.zeros((m, n), order="F")
+    oracle.dgemm("N", "N", m, n, k, 1.0, Ah, m, Bh, k, 0.0, ref, m, dd=True)
+    got, info = run_ozaki(handle, "N", "N", m, n, k, 1.0, Ah, Bh, 0.0, np.zeros((m, n), order="F"), 0)
+    err = oracle.rel_fro(got, ref)
+    assert err <= 1e-14, (err, info)
+    assert err <= info["error_bound"], (err, info)
+    U = oracle.uniform(m, k, 1)
+    _, info_u = run_ozaki(handle, "N", "N", m, n, k, 1.0, U, oracle.uniform(k, n, 2), 0.0,
+                          np.zeros((m, n), order="F"), 0)
+    assert info["count"] > info_u["count"], (info, info_u)
+
+
 def test_nonfinite_inputs_fall_back_to_fp64(handle):
     m, n, k = 40, 30, 50
     A, B = oracle.uniform(m, k, 1), oracle.uniform(k, n, 2)
@@ -213,4 +239,4 @@ def test_invalid_arguments(handle):
     with pytest.raises(ca.CosmaAmdError, match="method"):
         handle.dgemm_ozaki("N", "N", 4, 4, 4, 1.0, 16, 4, 16, 4, 0.0, 16, 4, method=5)
     with pytest.raises(ca.CosmaAmdError, match="moduli"):
-        handle.dgemm_ozaki("N", "N", 4, 4, 4, 1.0, 16, 4, 16, 4, 0.0, 16, 4, method=0, count=17)
+        handle.dgemm_ozaki("N", "N", 4, 4, 4, 1.0, 16, 4, 16, 4, 0.0, 16, 4, method=0, count=21)
