@@ -15,6 +15,8 @@
 //           both the read of src and the write of dst walk memory contiguously
 //           (rows of the tile are read along src columns and written along
 //           dst columns); the pad spreads the column reads over all banks.
+//           With aligned pointers and even leading dimensions the global
+//           accesses are 16 bytes wide (two consecutive elements).
 // alpha == 1 and beta == 0 copies values bit-for-bit (no arithmetic at all);
 // beta == 0 never reads dst.
 #include "common.hpp"
@@ -87,6 +89,75 @@ __global__ __launch_bounds__(TNT) void transpose_kernel(int64_t rows, int64_t co
     }
 }
 
+// Same transpose with 16-byte global accesses: every load takes two
+// consecutive src elements of one src column, every store writes two
+// consecutive dst elements of one dst column.  Needs 16-byte aligned
+// pointers and even leading dimensions; tiles that cross the matrix edge take
+// the element-wise path.
+__global__ __launch_bounds__(TNT) void transpose_vec_kernel(int64_t rows, int64_t cols, double alpha,
+                                                            const double *__restrict__ src, int64_t lds,
+                                                            double beta, double *__restrict__ dst,
+                                                            int64_t ldd, int mode) {
+    __shared__ double tile[TT][TT + 1];
+    const int t = threadIdx.x;
+    const int64_t i0 = (int64_t)blockIdx.x * TT;   // dst rows / src cols
+    for (int64_t j0 = (int64_t)blockIdx.y * TT; j0 < cols; j0 += (int64_t)gridDim.y * TT) {
+        const bool full = (i0 + TT <= rows) && (j0 + TT <= cols);
+        if (full) {
+            f64x2 v[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {     // src column i0 + il, rows j0 + c2, c2 + 1
+                const int idx = t + TNT * q, il = idx >> 5, c2 = (idx & 31) * 2;
+                v[q] = *reinterpret_cast<const f64x2 *>(src + (j0 + c2) + (i0 + il) * lds);
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int idx = t + TNT * q, il = idx >> 5, c2 = (idx & 31) * 2;
+                tile[il][c2] = v[q].x;
+                tile[il][c2 + 1] = v[q].y;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {     // dst column j0 + jl, rows i0 + r2, r2 + 1
+                const int idx = t + TNT * q, jl = idx >> 5, r2 = (idx & 31) * 2;
+                f64x2 *d = reinterpret_cast<f64x2 *>(dst + (i0 + r2) + (j0 + jl) * ldd);
+                f64x2 o;
+                if (mode == 0) {
+                    o.x = tile[r2][jl];
+                    o.y = tile[r2 + 1][jl];
+                } else if (mode == 1) {
+                    o.x = alpha * tile[r2][jl];
+                    o.y = alpha * tile[r2 + 1][jl];
+                } else {
+                    const f64x2 old = *d;
+                    o.x = fma(beta, old.x, alpha * tile[r2][jl]);
+                    o.y = fma(beta, old.y, alpha * tile[r2 + 1][jl]);
+                }
+                *d = o;
+            }
+        } else {
+            const int tx = t & 63, ty = t >> 6;
+#pragma unroll
+            for (int r = 0; r < TT / 4; ++r) {
+                const int il = ty + 4 * r;
+                const int64_t i = i0 + il, j = j0 + tx;
+                if (i < rows && j < cols) tile[il][tx] = src[j + i * lds];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < TT / 4; ++r) {
+                const int jl = ty + 4 * r;
+                const int64_t i = i0 + tx, j = j0 + jl;
+                if (i < rows && j < cols) {
+                    double *d = dst + i + j * ldd;
+                    *d = combine(tile[tx][jl], alpha, beta, d, mode);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
 inline bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
 
 }  // namespace
@@ -115,8 +186,13 @@ int launch_block_transform(char op, int64_t rows, int64_t cols, double alpha,
             set_error("block_transform: %lld rows exceed the grid limit", (long long)rows);
             return COSMA_AMD_ERR_UNSUPPORTED;
         }
-        hipLaunchKernelGGL(transpose_kernel, dim3((unsigned)bx, (unsigned)by), dim3(TNT), 0, s,
-                           rows, cols, alpha, src, lds, beta, dst, ldd, mode);
+        const bool vec = aligned16(src) && aligned16(dst) && (lds % 2 == 0) && (ldd % 2 == 0);
+        if (vec)
+            hipLaunchKernelGGL(transpose_vec_kernel, dim3((unsigned)bx, (unsigned)by), dim3(TNT), 0, s,
+                               rows, cols, alpha, src, lds, beta, dst, ldd, mode);
+        else
+            hipLaunchKernelGGL(transpose_kernel, dim3((unsigned)bx, (unsigned)by), dim3(TNT), 0, s,
+                               rows, cols, alpha, src, lds, beta, dst, ldd, mode);
     }
     CA_HIP(hipGetLastError());
     return COSMA_AMD_OK;
