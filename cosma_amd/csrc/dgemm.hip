@@ -171,7 +171,10 @@ struct TileSmem {
 // kernel (one tile per workgroup) and by the grouped kernel (a persistent
 // loop over the tiles of many problems); the caller provides the LDS tiles
 // and must not touch them while this runs.
-template <int BM, int BN, bool TA, bool TB, bool VEC>
+// EDGE: skip the MFMAs of 16 x 16 blocks that lie wholly outside C (tiles on
+// the bottom / right edge of small problems); their waves leave the SIMD to
+// the other workgroups resident on the CU.
+template <int BM, int BN, bool TA, bool TB, bool VEC, bool EDGE = false>
 __device__ __forceinline__ void gemm_tile(
     int64_t M, int64_t N, int64_t K, double alpha,
     const double *__restrict__ A, int64_t lda,
@@ -215,17 +218,33 @@ __device__ __forceinline__ void gemm_tile(
 #pragma unroll
         for (int j = 0; j < FN; ++j) bf[h][j] = sb[st][kp][bbase + 16 * j];
     };
+    // blocks of this wave that hold rows / columns of C (all of them unless EDGE)
+    const int64_t wrow0 = m0 + wm * (BM / 2), wcol0 = n0 + wn * (BN / 2);
+    const int rows_in = EDGE ? (int)max((int64_t)0, min((int64_t)FM, (M - wrow0 + 15) / 16)) : FM;
+    const int cols_in = EDGE ? (int)max((int64_t)0, min((int64_t)FN, (N - wcol0 + 15) / 16)) : FN;
+    const bool whole = !EDGE || (rows_in == FM && cols_in == FN);
     auto mfma_frags = [&](int h) {
+        if (whole) {
 #pragma unroll
-        for (int i = 0; i < FM; ++i)
+            for (int i = 0; i < FM; ++i)
 #pragma unroll
-            for (int j = 0; j < FN; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[h][i].x, bf[h][j].x, acc[i][j], 0, 0, 0);
+                for (int j = 0; j < FN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[h][i].x, bf[h][j].x, acc[i][j], 0, 0, 0);
 #pragma unroll
-        for (int i = 0; i < FM; ++i)
+            for (int i = 0; i < FM; ++i)
 #pragma unroll
-            for (int j = 0; j < FN; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[h][i].y, bf[h][j].y, acc[i][j], 0, 0, 0);
+                for (int j = 0; j < FN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[h][i].y, bf[h][j].y, acc[i][j], 0, 0, 0);
+        } else {
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j)
+                    if (i < rows_in && j < cols_in) {
+                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[h][i].x, bf[h][j].x, acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[h][i].y, bf[h][j].y, acc[i][j], 0, 0, 0);
+                    }
+        }
     };
 
     // LDS stores per thread and step, MFMAs per k-sub-step
@@ -448,7 +467,7 @@ __global__ __launch_bounds__(NT) void dgemm_grouped_kernel(const GroupedProblem 
         const double alpha = scale_only ? 0.0 : q.alpha;
         __syncthreads();   // the previous tile's last LDS reads are done
 #define CA_TILE(X, Y, V)                                                                              \
-    gemm_tile<BM, BN, X, Y, V>(q.m, q.n, K, alpha, q.a, q.lda, q.b, q.ldb, q.beta, q.c, q.ldc, q.vec_c, m0, n0, sm)
+    gemm_tile<BM, BN, X, Y, V, true>(q.m, q.n, K, alpha, q.a, q.lda, q.b, q.ldb, q.beta, q.c, q.ldc, q.vec_c, m0, n0, sm)
         const int f = q.flags;   // bit 0: A transposed, bit 1: B transposed, bit 2: 16-byte loads
         switch (f) {
         case 0: CA_TILE(false, false, false); break;
