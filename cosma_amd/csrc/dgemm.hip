@@ -56,7 +56,7 @@ constexpr int NXCD = 8;      // XCDs on MI355X
 //     work item idx -> row pair rp = idx % (ROWS/2), k-pair kp = idx / (ROWS/2);
 //     two loads (k = 2kp and 2kp+1), each covering rows 2rp, 2rp+1.
 //   otherwise: element (r, k) at x[k + r * ld]   (A 'T', B 'N')
-//     work item idx -> k-pair kp = idx % KP, row r = idx / KP; one load.
+//     work item idx -> (row, k-pair) by kmap; one load.
 // Rows at or beyond R are clamped to a valid row: they only feed rows (or
 // columns) of C that the epilogue never stores, so their values are
 // irrelevant and no branch is needed.  Only k must be exact: full steps
@@ -70,6 +70,19 @@ struct TileIO {
     static constexpr int PER_T = ITEMS / NT;           // work items per thread
     static constexpr int NREG = RowContig ? 2 * PER_T : PER_T;
     static_assert(ITEMS % NT == 0, "tile must divide evenly over the workgroup");
+    static_assert(RowContig || ROWS % 16 == 0, "k-contiguous tiles need 16-row groups");
+
+    // k-contiguous operands: work item idx -> (row, k-pair).  Eight lanes read
+    // the 128 bytes of one row (k-pairs 0..7), and each group of 16 lanes
+    // (one LDS store pass) takes rows r and r + 8, so the 16 stores land on
+    // 16 different 16-byte bank groups of the padded [kp][row] tile
+    // (address mod 256 = 16 * (kp + row) mod 256).  Wave-instruction pairs
+    // (W even / odd) cover 16 consecutive rows.
+    __device__ __forceinline__ static void kmap(int idx, int &row, int &kp) {
+        const int lane = idx & 63, W = idx >> 6;
+        kp = lane & 7;
+        row = 16 * (W >> 1) + 4 * (W & 1) + (lane >> 4) + 8 * ((lane >> 3) & 1);
+    }
 
     __device__ __forceinline__ static void load(const double *__restrict__ x, int64_t ld,
                                                 int64_t r0, int64_t k0, int64_t R,
@@ -93,8 +106,10 @@ struct TileIO {
                     v[2 * q + 1].y = x[rb + (k + 1) * ld];
                 }
             } else {
-                const int64_t k = k0 + 2 * (idx % KP);
-                const int64_t r = min(r0 + idx / KP, R - 1);
+                int row, kp;
+                kmap(idx, row, kp);
+                const int64_t k = k0 + 2 * kp;
+                const int64_t r = min(r0 + row, R - 1);
                 if (VEC) {
                     v[q] = *reinterpret_cast<const f64x2 *>(x + k + r * ld);
                 } else {
@@ -123,8 +138,10 @@ struct TileIO {
                     v[2 * q + dk].y = kin ? x[rb + (k + dk) * ld] : 0.0;
                 }
             } else {
-                const int64_t k = k0 + 2 * (idx % KP);
-                const int64_t r = min(r0 + idx / KP, R - 1);
+                int row, kp;
+                kmap(idx, row, kp);
+                const int64_t k = k0 + 2 * kp;
+                const int64_t r = min(r0 + row, R - 1);
                 v[q].x = (k < K) ? x[k + r * ld] : 0.0;
                 v[q].y = (k + 1 < K) ? x[k + 1 + r * ld] : 0.0;
             }
@@ -145,7 +162,9 @@ struct TileIO {
                 tile[kp][r] = lo;
                 tile[kp][r + 1] = hi;
             } else {
-                tile[idx % KP][idx / KP] = v[q];
+                int row, kp;
+                kmap(idx, row, kp);
+                tile[kp][row] = v[q];
             }
         }
     }
