@@ -14,10 +14,13 @@
 //     in a "k-pair" layout  tile[kp][row] = (x[row][2kp], x[row][2kp+1])  so
 //     that one 16-byte LDS read per lane feeds two v_mfma_f64_16x16x4f64
 //     (lane group g = lane/16 supplies k = 2kp and 2kp+1 of the same pair).
-//     Each kp-row of the tile is padded by one 16-byte entry so that the
-//     k-contiguous global-load pattern (8 lanes per matrix row) writes LDS
-//     without bank conflicts, while the MFMA reads (16 lanes on 16
-//     consecutive rows of one kp) stay conflict-free too.
+//     Rows are stored XOR-swizzled, tile[kp][row ^ kp] (kp < 8 flips the low
+//     three row bits), with no padding: the MFMA reads (16 lanes on 16
+//     consecutive rows of one kp) and the k-contiguous stores (eight lanes
+//     per row, one per kp) both hit distinct bank groups.  Measured with
+//     SQ_LDS_BANK_CONFLICT on gfx950: unpadded + swizzled reads and stores
+//     are conflict-free, while a one-entry row pad (the classic fix) costs
+//     0.76 conflict cycles per fragment read.
 //     The order in which k is summed is a permutation of 0..K-1; it only
 //     changes rounding, not the mathematical result.
 //   * LDS is double-buffered: the global loads of step t+1 are issued into
@@ -148,7 +151,7 @@ struct TileIO {
         }
     }
 
-    __device__ __forceinline__ static void store(f64x2 (*tile)[ROWS + 1], const f64x2 (&v)[NREG]) {
+    __device__ __forceinline__ static void store(f64x2 (*tile)[ROWS], const f64x2 (&v)[NREG]) {
         const int t = threadIdx.x;
 #pragma unroll
         for (int q = 0; q < PER_T; ++q) {
@@ -159,12 +162,12 @@ struct TileIO {
                 f64x2 lo, hi;
                 lo.x = v[2 * q].x;  lo.y = v[2 * q + 1].x;   // row r:   (k, k+1)
                 hi.x = v[2 * q].y;  hi.y = v[2 * q + 1].y;   // row r+1: (k, k+1)
-                tile[kp][r] = lo;
-                tile[kp][r + 1] = hi;
+                tile[kp][r ^ kp] = lo;
+                tile[kp][(r + 1) ^ kp] = hi;
             } else {
                 int row, kp;
                 kmap(idx, row, kp);
-                tile[kp][row] = v[q];
+                tile[kp][row ^ kp] = v[q];
             }
         }
     }
@@ -181,8 +184,8 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t T) {
 
 template <int BM, int BN>
 struct TileSmem {
-    f64x2 a[2][KP][BM + 1];
-    f64x2 b[2][KP][BN + 1];
+    f64x2 a[2][KP][BM];
+    f64x2 b[2][KP][BN];
 };
 
 // One BM x BN tile of C at (m0, n0): the whole k loop and the epilogue,
@@ -233,9 +236,9 @@ __device__ __forceinline__ void gemm_tile(
     auto read_frags = [&](int st, int ks, int h) {
         const int kp = 4 * ks + g;
 #pragma unroll
-        for (int i = 0; i < FM; ++i) af[h][i] = sa[st][kp][abase + 16 * i];
+        for (int i = 0; i < FM; ++i) af[h][i] = sa[st][kp][(abase + 16 * i) ^ kp];
 #pragma unroll
-        for (int j = 0; j < FN; ++j) bf[h][j] = sb[st][kp][bbase + 16 * j];
+        for (int j = 0; j < FN; ++j) bf[h][j] = sb[st][kp][(bbase + 16 * j) ^ kp];
     };
     // blocks of this wave that hold rows / columns of C (all of them unless EDGE)
     const int64_t wrow0 = m0 + wm * (BM / 2), wcol0 = n0 + wn * (BN / 2);
@@ -524,9 +527,10 @@ int launch_dgemm_cfg(int cfg, int group_m, char ta, char tb, int64_t m, int64_t 
     if (m == 0 || n == 0) return COSMA_AMD_OK;
     if (k == 0 || alpha == 0.0) return launch_scale(m, n, beta, c, ldc, s);
     if (cfg == 0) {
-        // Large tiles need enough of them to fill 256 CUs.
-        const int64_t t256 = ((m + 255) / 256) * ((n + 127) / 128);
-        cfg = (t256 >= 512) ? 2 : 1;
+        // Large tiles need enough of them to fill 256 CUs; of the two large
+        // shapes 128 x 256 measured faster for NN and TN at 8192^3 and 16384^3.
+        const int64_t t_large = ((m + 127) / 128) * ((n + 255) / 256);
+        cfg = (t_large >= 512) ? 3 : 1;
     }
     switch (cfg) {
     case 1: return launch_cfg<128, 128>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, group_m > 0 ? group_m : 8, s);
