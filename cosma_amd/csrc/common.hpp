@@ -81,6 +81,7 @@ struct cosma_amd_handle_s {
     int64_t mem_limit = 0;              // device bytes the streamer may use (0 = 80% of free)
     cosma_amd::DevBuf dA, dB, dC;       // resident operand copies / ring slots
     cosma_amd::DevBuf dX;               // product before the layout epilogue
+    cosma_amd::DevBuf dC0;              // ring of incoming C tiles (deferred beta * C)
     cosma_amd::DevBuf ozaki_ws;         // split-precision workspace
     cosma_amd::DevBuf ozaki_aux;        // split-precision row statistics and exponents
     std::vector<hipEvent_t> events;     // event pool
@@ -132,6 +133,8 @@ int launch_dgemm_cfg(int cfg, int group_m, char ta, char tb, int64_t m, int64_t 
 void set_dgemm_tuning(int cfg, int group_m);
 int launch_scale(int64_t m, int64_t n, double beta, double *c, int64_t ldc,
                  hipStream_t s);
+int launch_add_scaled(int64_t m, int64_t n, double beta, const double *x, int64_t ldx, double *c,
+                      int64_t ldc, hipStream_t s);
 int launch_fill_uniform(double *x, int64_t rows, int64_t cols, int64_t ld, uint64_t seed,
                         uint64_t offset, hipStream_t s);
 int stream_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k,

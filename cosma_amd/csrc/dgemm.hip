@@ -414,14 +414,27 @@ __global__ __launch_bounds__(NT) void dgemm_mfma_kernel(
 }
 
 // C <- beta * C; beta == 0 writes zeros without reading C.
+// Elementwise kernels over an M x N column-major block: threads run down the
+// rows of a column (coalesced), blocks stride over the columns.
 __global__ void scale_kernel(int64_t M, int64_t N, double beta, double *__restrict__ C, int64_t ldc) {
-    const int64_t total = M * N;
-    for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
-         idx += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t col = idx / M, row = idx - col * M;
+    const int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (row >= M) return;
+    for (int64_t col = blockIdx.y; col < N; col += gridDim.y) {
         double *p = C + row + col * ldc;
         *p = (beta == 0.0) ? 0.0 : beta * *p;
     }
+}
+
+// C += beta * X over an M x N block (X leading dimension ldx).
+__global__ void add_scaled_kernel(int64_t M, int64_t N, double beta, const double *__restrict__ X,
+                                  int64_t ldx, double *__restrict__ C, int64_t ldc) {
+    const int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (row >= M) return;
+    for (int64_t col = blockIdx.y; col < N; col += gridDim.y) C[row + col * ldc] += beta * X[row + col * ldx];
+}
+
+inline dim3 elementwise_grid(int64_t m, int64_t n) {
+    return dim3((unsigned)((m + 255) / 256), (unsigned)std::min<int64_t>(n, 2048));
 }
 
 inline bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
@@ -509,10 +522,15 @@ __global__ __launch_bounds__(NT) void dgemm_grouped_kernel(const GroupedProblem 
 
 int launch_scale(int64_t m, int64_t n, double beta, double *c, int64_t ldc, hipStream_t s) {
     if (m == 0 || n == 0 || beta == 1.0) return COSMA_AMD_OK;
-    const int64_t total = m * n;
-    int64_t blocks = (total + 255) / 256;
-    if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(scale_kernel, dim3((unsigned)blocks), dim3(256), 0, s, m, n, beta, c, ldc);
+    hipLaunchKernelGGL(scale_kernel, elementwise_grid(m, n), dim3(256), 0, s, m, n, beta, c, ldc);
+    CA_HIP(hipGetLastError());
+    return COSMA_AMD_OK;
+}
+
+int launch_add_scaled(int64_t m, int64_t n, double beta, const double *x, int64_t ldx, double *c,
+                      int64_t ldc, hipStream_t s) {
+    if (m == 0 || n == 0 || beta == 0.0) return COSMA_AMD_OK;
+    hipLaunchKernelGGL(add_scaled_kernel, elementwise_grid(m, n), dim3(256), 0, s, m, n, beta, x, ldx, c, ldc);
     CA_HIP(hipGetLastError());
     return COSMA_AMD_OK;
 }

@@ -11,19 +11,25 @@
 // run on separate streams and are ordered only by events:
 //   * one host->device stream uploads operand blocks, in the order the tiles
 //     need them;
-//   * compute streams (COSMA_GPU_STREAMS of them, round-robin over C tiles)
-//     run the FP64 MFMA DGEMM of each (tile, panel) as soon as that block's
-//     upload event has fired: panel 0 applies the caller's beta (and reads C
-//     only if beta != 0), later panels accumulate with beta = 1;
+//   * compute streams run the FP64 MFMA DGEMM of each (tile, panel) as soon
+//     as that block's upload event has fired; the products of one tile
+//     accumulate in order on one stream (the first with the tile's starting
+//     beta, the rest with 1);
 //   * one device->host stream returns each finished C tile while later tiles
 //     are still being uploaded and multiplied (PCIe is full duplex).
 // Resident mode (the default when the operands fit in the device-memory
 // budget — 288 GB of HBM makes this the common case): every block of A and B
 // is uploaded once into a device copy of the whole operand and reused by all
 // tiles that need it, so the host link carries A, B, (C) once instead of
-// re-streaming panels per tile.  Bounded mode (operands larger than the
-// budget): Tiled-MM's scheme — per tile, panels pass through two device
-// slots per operand whose reuse waits on the compute that last read them.
+// re-streaming panels per tile.  Tiles are visited in growing squares; the
+// first half of the panels runs panel-major, the rest tile by tile; a host C
+// with beta != 0 is uploaded last per tile and added as beta * C after the
+// tile's products, so the link first carries the blocks that start compute;
+// all products run on one compute stream in upload order.
+// Bounded mode (operands larger than the budget): Tiled-MM's scheme — per
+// tile, panels pass through two device slots per operand whose reuse waits
+// on the compute that last read them, panel 0 applying beta to the C tile,
+// with the tiles round-robin over the COSMA_GPU_STREAMS compute streams.
 //
 // Host memory.  Pinned host buffers (hipHostMalloc) are copied directly by DMA.
 // Pageable buffers go through a ring of pinned staging slots: the host packs
@@ -260,7 +266,7 @@ int upload_op_block(cosma_amd_handle_t h, const Operand &X, double *dst, int64_t
 
 void release_stream_state(cosma_amd_handle_t h) {
     hipSetDevice(h->device);
-    for (DevBuf *b : {&h->dA, &h->dB, &h->dC, &h->dX, &h->ozaki_ws, &h->ozaki_aux}) {
+    for (DevBuf *b : {&h->dA, &h->dB, &h->dC, &h->dC0, &h->dX, &h->ozaki_ws, &h->ozaki_aux}) {
         if (b->ptr) hipFree(b->ptr);
         b->ptr = nullptr;
         b->bytes = 0;
@@ -276,6 +282,273 @@ void release_stream_state(cosma_amd_handle_t h) {
     h->events.clear();
     h->events_used = 0;
     h->dev_c = nullptr;
+}
+
+// Visiting order of the nTm x nTn C tiles: squares growing from tile (0,0).
+// After s block rows of op(A) and s block columns of op(B) have arrived, s*s
+// tiles can run, which front-loads the work each upload enables.
+static std::vector<std::pair<int64_t, int64_t>> shell_order(int64_t nTm, int64_t nTn) {
+    std::vector<std::pair<int64_t, int64_t>> out;
+    out.reserve((size_t)(nTm * nTn));
+    for (int64_t s = 0; s < std::max(nTm, nTn); ++s) {
+        if (s < nTm)
+            for (int64_t j = 0; j < std::min(s, nTn); ++j) out.emplace_back(s, j);
+        if (s < nTn)
+            for (int64_t i = 0; i <= std::min(s, nTm - 1); ++i) out.emplace_back(i, s);
+    }
+    return out;
+}
+
+// Resident schedule: each block of A and B crosses the host link once, into a
+// device copy of the whole operand, and is reused by every tile that needs it.
+// The first floor(nTk/2) panels run panel-major over all tiles (the blocks of
+// panel p enable every tile's product p, so few uploads start much compute);
+// the remaining ceil(nTk/2) panels run tile by tile, each tile then taking its
+// deferred beta * C and leaving for the host while later tiles compute, so
+// the returns overlap the compute instead of trailing it.
+static int stream_resident(cosma_amd_handle_t h, const Operand &A, const Operand &B, Kind kc, int64_t m, int64_t n,
+                           int64_t k, double alpha, double beta, double *c, int64_t ldc, bool copy_c_back,
+                           int64_t tm, int64_t tn, int64_t tk, int64_t nTm, int64_t nTn, int64_t nTk,
+                           bool defer_c, int64_t R) {
+    int rc;
+    double *dA = const_cast<double *>(A.host), *dB = const_cast<double *>(B.host), *dC = c, *dC0 = nullptr;
+    int64_t ldA = A.ld, ldB = B.ld, ldC = ldc;
+    if (A.kind != DEVICE) {
+        if ((rc = grow(h->dA, (size_t)A.srows * A.scols * 8))) return rc;
+        dA = static_cast<double *>(h->dA.ptr);
+        ldA = A.srows;
+    }
+    if (B.kind != DEVICE) {
+        if ((rc = grow(h->dB, (size_t)B.srows * B.scols * 8))) return rc;
+        dB = static_cast<double *>(h->dB.ptr);
+        ldB = B.srows;
+    }
+    if (kc != DEVICE) {
+        if ((rc = grow(h->dC, (size_t)m * n * 8))) return rc;
+        dC = static_cast<double *>(h->dC.ptr);
+        ldC = m;
+    }
+    if (defer_c) {
+        if ((rc = grow(h->dC0, (size_t)R * tm * tn * 8))) return rc;
+        dC0 = static_cast<double *>(h->dC0.ptr);
+    }
+
+    std::vector<hipEvent_t> evA((size_t)(nTm * nTk), nullptr), evB((size_t)(nTk * nTn), nullptr);
+    std::vector<hipEvent_t> slot_free((size_t)R, nullptr);
+    const auto order = shell_order(nTm, nTn);
+    const int64_t tail = (nTk + 1) / 2, head = nTk - tail;
+    // Products run on one stream in the order their blocks are uploaded (the
+    // upload stream is FIFO, so a second compute stream would only split the
+    // GPU between kernels whose inputs arrive in the same order).
+    hipStream_t s = h->streams[0];
+    const double beta0 = defer_c ? 0.0 : beta;   // beta of each tile's first product
+    TRACE("resident: %lld panel-major panels, %lld per tile, C %s", (long long)head, (long long)tail,
+          defer_c ? "deferred" : "in place");
+
+    // The first product would wait for two whole blocks: upload them in
+    // k-chunks and multiply chunk by chunk, so the compute starts after the
+    // first chunks arrive.  Later products wait for the whole blocks.
+    bool first_split = false;
+    if (nTk > 0 && (A.kind != DEVICE || B.kind != DEVICE)) {
+        const int64_t i = order[0].first, j = order[0].second;
+        const int64_t mi = std::min(tm, m), nj = std::min(tn, n), kp = std::min(tk, k);
+        const int64_t nsub = std::max<int64_t>(1, std::min<int64_t>(4, kp / 256));
+        if (nsub > 1) {
+            const int64_t sub = (kp + nsub - 1) / nsub;
+            const int64_t r0 = i * tm, c0 = j * tn;
+            hipEvent_t ev = nullptr;
+            for (int64_t kk = 0; kk < kp; kk += sub) {
+                const int64_t kq = std::min(sub, kp - kk);
+                if (A.kind != DEVICE && (rc = upload_op_block(h, A, dA, ldA, r0, kk, mi, kq, false))) return rc;
+                if (B.kind != DEVICE && (rc = upload_op_block(h, B, dB, ldB, kk, c0, kq, nj, false))) return rc;
+                if ((rc = record(h, h->h2d, &ev))) return rc;
+                CA_HIP(hipStreamWaitEvent(s, ev, 0));
+                rc = launch_dgemm(A.t, B.t, mi, nj, kq, alpha, op_block(dA, ldA, A.t, r0, kk), ldA,
+                                  op_block(dB, ldB, B.t, kk, c0), ldB, kk == 0 ? beta0 : 1.0,
+                                  dC + r0 + c0 * ldC, ldC, s);
+                if (rc) return rc;
+            }
+            if (A.kind != DEVICE) evA[(size_t)(i * nTk)] = ev;
+            if (B.kind != DEVICE) evB[(size_t)j] = ev;
+            first_split = true;
+            TRACE("first product in %lld k-chunks of %lld", (long long)nsub, (long long)sub);
+        }
+    }
+
+    // Product of panel p into tile (i, j); blocks are uploaded on their first
+    // use and waited for through their upload events.
+    auto product = [&](int64_t i, int64_t j, int64_t p) -> int {
+        if (first_split && p == 0 && i == order[0].first && j == order[0].second) return COSMA_AMD_OK;
+        const int64_t mi = std::min(tm, m - i * tm), nj = std::min(tn, n - j * tn);
+        const int64_t r0 = i * tm, c0 = j * tn, k0 = p * tk, kp = std::min(tk, k - k0);
+        int rc2;
+        if (A.kind != DEVICE) {
+            hipEvent_t &e = evA[(size_t)(i * nTk + p)];
+            if (!e) {
+                if ((rc2 = upload_op_block(h, A, dA, ldA, r0, k0, mi, kp, false))) return rc2;
+                if ((rc2 = record(h, h->h2d, &e))) return rc2;
+            }
+            CA_HIP(hipStreamWaitEvent(s, e, 0));
+        }
+        if (B.kind != DEVICE) {
+            hipEvent_t &e = evB[(size_t)(p * nTn + j)];
+            if (!e) {
+                if ((rc2 = upload_op_block(h, B, dB, ldB, k0, c0, kp, nj, false))) return rc2;
+                if ((rc2 = record(h, h->h2d, &e))) return rc2;
+            }
+            CA_HIP(hipStreamWaitEvent(s, e, 0));
+        }
+        return launch_dgemm(A.t, B.t, mi, nj, kp, alpha, op_block(dA, ldA, A.t, r0, k0), ldA,
+                            op_block(dB, ldB, B.t, k0, c0), ldB, p > 0 ? 1.0 : beta0, dC + r0 + c0 * ldC,
+                            ldC, s);
+    };
+
+    for (int64_t p = 0; p < head; ++p) {
+        for (const auto &ij : order) {
+            if ((rc = product(ij.first, ij.second, p))) return rc;
+        }
+    }
+    for (size_t t = 0; t < order.size(); ++t) {
+        const int64_t i = order[t].first, j = order[t].second;
+        const int64_t mi = std::min(tm, m - i * tm), nj = std::min(tn, n - j * tn);
+        const int64_t r0 = i * tm, c0 = j * tn;
+        double *Ct = dC + r0 + c0 * ldC;
+        const double *Ch = c + r0 + c0 * ldc;
+        TRACE("tile (%lld,%lld)", (long long)i, (long long)j);
+        for (int64_t p = head; p < nTk; ++p) {
+            if ((rc = product(i, j, p))) return rc;
+        }
+        if (nTk == 0) {   // k == 0 or alpha == 0 (C is on the host here)
+            if (beta != 0.0) {
+                if ((rc = upload(h, Ct, ldC, Ch, ldc, mi, nj, kc))) return rc;
+                hipEvent_t ev;
+                if ((rc = record(h, h->h2d, &ev))) return rc;
+                CA_HIP(hipStreamWaitEvent(s, ev, 0));
+            }
+            if ((rc = launch_scale(mi, nj, beta, Ct, ldC, s))) return rc;
+        }
+        if (defer_c) {
+            const size_t q = t % (size_t)R;
+            double *slot = dC0 + q * (size_t)(tm * tn);
+            if (slot_free[q]) CA_HIP(hipStreamWaitEvent(h->h2d, slot_free[q], 0));
+            if ((rc = upload(h, slot, tm, Ch, ldc, mi, nj, kc))) return rc;
+            hipEvent_t ev;
+            if ((rc = record(h, h->h2d, &ev))) return rc;
+            CA_HIP(hipStreamWaitEvent(s, ev, 0));
+            if ((rc = launch_add_scaled(mi, nj, beta, slot, tm, Ct, ldC, s))) return rc;
+            if ((rc = record(h, s, &slot_free[q]))) return rc;
+        }
+        if (kc != DEVICE && copy_c_back) {
+            hipEvent_t done;
+            if ((rc = record(h, s, &done))) return rc;
+            CA_HIP(hipStreamWaitEvent(h->d2h, done, 0));
+            if ((rc = download(h, const_cast<double *>(Ch), ldc, Ct, ldC, mi, nj, kc))) return rc;
+        }
+    }
+    if (kc != DEVICE && !copy_c_back) {
+        h->dev_c = dC;
+        h->dev_c_ld = ldC;
+    }
+    return COSMA_AMD_OK;
+}
+
+// Bounded schedule (operands larger than the device budget): per tile, the
+// panels of op(A) and op(B) pass through two device slots per operand whose
+// reuse waits on the product that last read them; C tiles of host C use one
+// slot per compute stream, reused once the tile has left for the host.
+static int stream_bounded(cosma_amd_handle_t h, const Operand &A, const Operand &B, Kind kc, int64_t m, int64_t n,
+                   int64_t k, double alpha, double beta, double *c, int64_t ldc,
+                   int64_t tm, int64_t tn, int64_t tk, int64_t nTm, int64_t nTn, int64_t nTk) {
+    const int S = (int)h->streams.size();
+    int rc;
+    const size_t slotA = (size_t)tm * tk * 8, slotB = (size_t)tk * tn * 8, slotC = (size_t)tm * tn * 8;
+    if (A.kind != DEVICE && (rc = grow(h->dA, 2 * slotA))) return rc;
+    if (B.kind != DEVICE && (rc = grow(h->dB, 2 * slotB))) return rc;
+    if (kc != DEVICE && (rc = grow(h->dC, (size_t)S * slotC))) return rc;
+
+    hipEvent_t slot_free_A[2] = {nullptr, nullptr}, slot_free_B[2] = {nullptr, nullptr};
+    std::vector<hipEvent_t> c_slot_free(S, nullptr);
+    int64_t ring = 0;
+    for (int64_t i = 0; i < nTm; ++i) {
+        for (int64_t jj = 0; jj < nTn; ++jj) {
+            const int64_t j = (i % 2 == 0) ? jj : nTn - 1 - jj;     // snake order
+            const int64_t t = i * nTn + jj;
+            hipStream_t s = h->streams[t % S];
+            const int64_t mi = std::min(tm, m - i * tm), nj = std::min(tn, n - j * tn);
+            const int64_t r0 = i * tm, c0 = j * tn;
+            double *Ct;
+            int64_t ldCt;
+            if (kc == DEVICE) {
+                Ct = c + r0 + c0 * ldc;
+                ldCt = ldc;
+            } else {
+                Ct = static_cast<double *>(h->dC.ptr) + (size_t)(t % S) * tm * tn;
+                ldCt = tm;
+                // the slot's previous tile must have left the device before it is reused
+                if (c_slot_free[t % S]) {
+                    CA_HIP(hipStreamWaitEvent(h->h2d, c_slot_free[t % S], 0));
+                    CA_HIP(hipStreamWaitEvent(s, c_slot_free[t % S], 0));
+                }
+            }
+            TRACE("tile (%lld,%lld) stream %d C at %p ld %lld", (long long)i, (long long)j, (int)(t % S),
+                  (void *)Ct, (long long)ldCt);
+            if (kc != DEVICE && beta != 0.0) {
+                if ((rc = upload(h, Ct, ldCt, c + r0 + c0 * ldc, ldc, mi, nj, kc))) return rc;
+                hipEvent_t ev;
+                if ((rc = record(h, h->h2d, &ev))) return rc;
+                CA_HIP(hipStreamWaitEvent(s, ev, 0));
+            }
+            if (nTk == 0 && (rc = launch_scale(mi, nj, beta, Ct, ldCt, s))) return rc;
+            for (int64_t p = 0; p < nTk; ++p) {
+                const int64_t k0 = p * tk, kp = std::min(tk, k - k0);
+                const int q = ring % 2;
+                const double *Ap, *Bp;
+                int64_t ldAp = A.ld, ldBp = B.ld;
+                if (A.kind == DEVICE) {
+                    Ap = op_block(A.host, A.ld, A.t, r0, k0);
+                } else {
+                    double *slot = static_cast<double *>(h->dA.ptr) + (size_t)q * tm * tk;
+                    ldAp = (A.t == 'N') ? mi : kp;
+                    if (slot_free_A[q]) CA_HIP(hipStreamWaitEvent(h->h2d, slot_free_A[q], 0));
+                    if ((rc = upload_op_block(h, A, slot, ldAp, r0, k0, mi, kp, true))) return rc;
+                    hipEvent_t e;
+                    if ((rc = record(h, h->h2d, &e))) return rc;
+                    CA_HIP(hipStreamWaitEvent(s, e, 0));
+                    Ap = slot;
+                }
+                if (B.kind == DEVICE) {
+                    Bp = op_block(B.host, B.ld, B.t, k0, c0);
+                } else {
+                    double *slot = static_cast<double *>(h->dB.ptr) + (size_t)q * tk * tn;
+                    ldBp = (B.t == 'N') ? kp : nj;
+                    if (slot_free_B[q]) CA_HIP(hipStreamWaitEvent(h->h2d, slot_free_B[q], 0));
+                    if ((rc = upload_op_block(h, B, slot, ldBp, k0, c0, kp, nj, true))) return rc;
+                    hipEvent_t e;
+                    if ((rc = record(h, h->h2d, &e))) return rc;
+                    CA_HIP(hipStreamWaitEvent(s, e, 0));
+                    Bp = slot;
+                }
+                rc = launch_dgemm(A.t, B.t, mi, nj, kp, alpha, Ap, ldAp, Bp, ldBp, p == 0 ? beta : 1.0,
+                                  Ct, ldCt, s);
+                if (rc) return rc;
+                hipEvent_t e;
+                if ((rc = record(h, s, &e))) return rc;
+                if (A.kind != DEVICE) slot_free_A[q] = e;
+                if (B.kind != DEVICE) slot_free_B[q] = e;
+                ++ring;
+            }
+            if (kc != DEVICE) {
+                hipEvent_t done;
+                if ((rc = record(h, s, &done))) return rc;
+                CA_HIP(hipStreamWaitEvent(h->d2h, done, 0));
+                if ((rc = download(h, c + r0 + c0 * ldc, ldc, Ct, ldCt, mi, nj, kc))) return rc;
+                hipEvent_t freed;
+                if ((rc = record(h, h->d2h, &freed))) return rc;
+                c_slot_free[t % S] = freed;
+            }
+        }
+    }
+    return COSMA_AMD_OK;
 }
 
 int stream_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k,
@@ -301,7 +574,11 @@ int stream_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, i
     const int64_t tk = std::max<int64_t>(1, std::min(h->tile_k, k));
     const int64_t nTm = (m + tm - 1) / tm, nTn = (n + tn - 1) / tn;
     const int64_t nTk = scale_only ? 0 : (k + tk - 1) / tk;
-    const int S = (int)h->streams.size();
+    // host C with beta != 0: beta * C is added after the tile's last panel
+    // from a ring of R device slots, so that C uploads queue behind the
+    // operand blocks that start the compute.
+    const bool defer_c = kc != DEVICE && beta != 0.0 && nTk > 0;
+    const int64_t R = std::min<int64_t>(4, nTm * nTn);
 
     size_t free_b = 0, total_b = 0;
     CA_HIP(hipMemGetInfo(&free_b, &total_b));
@@ -309,7 +586,8 @@ int stream_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, i
     const size_t needA = A.kind == DEVICE ? 0 : (size_t)A.srows * A.scols * 8;
     const size_t needB = B.kind == DEVICE ? 0 : (size_t)B.srows * B.scols * 8;
     const size_t needC = kc == DEVICE ? 0 : (size_t)m * n * 8;
-    const bool resident = needA + needB + needC <= limit;
+    const size_t needC0 = defer_c ? (size_t)R * tm * tn * 8 : 0;
+    const bool resident = needA + needB + needC + needC0 <= limit;
     h->last_mode = resident ? 1 : 2;
     TRACE("dgemm_host %c%c m=%lld n=%lld k=%lld alpha=%g beta=%g kinds A=%d B=%d C=%d tiles %lldx%lldx%lld "
           "grid %lldx%lldx%lld mode=%s", ta, tb, (long long)m, (long long)n, (long long)k, alpha, beta,
@@ -320,154 +598,12 @@ int stream_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, i
                   "(%zu bytes over a budget of %zu)", needA + needB + needC, limit);
         return COSMA_AMD_ERR_UNSUPPORTED;
     }
-    int rc;
-
-    // ---- device storage
-    double *dA = const_cast<double *>(a), *dB = const_cast<double *>(b), *dC = c;
-    int64_t ldA = lda, ldB = ldb, ldC = ldc;
-    if (resident) {
-        if (A.kind != DEVICE) {
-            if ((rc = grow(h->dA, needA))) return rc;
-            dA = static_cast<double *>(h->dA.ptr);
-            ldA = A.srows;
-        }
-        if (B.kind != DEVICE) {
-            if ((rc = grow(h->dB, needB))) return rc;
-            dB = static_cast<double *>(h->dB.ptr);
-            ldB = B.srows;
-        }
-        if (kc != DEVICE) {
-            if ((rc = grow(h->dC, needC))) return rc;
-            dC = static_cast<double *>(h->dC.ptr);
-            ldC = m;
-        }
-    } else {
-        // two slots per streamed operand, one C slot per compute stream
-        const size_t slotA = (size_t)tm * tk * 8, slotB = (size_t)tk * tn * 8, slotC = (size_t)tm * tn * 8;
-        if (A.kind != DEVICE && (rc = grow(h->dA, 2 * slotA))) return rc;
-        if (B.kind != DEVICE && (rc = grow(h->dB, 2 * slotB))) return rc;
-        if (kc != DEVICE && (rc = grow(h->dC, (size_t)S * slotC))) return rc;
-    }
-
-    std::vector<hipEvent_t> evA(resident ? nTm * nTk : 0, nullptr), evB(resident ? nTk * nTn : 0, nullptr);
-    hipEvent_t slot_free_A[2] = {nullptr, nullptr}, slot_free_B[2] = {nullptr, nullptr};
-    std::vector<hipEvent_t> c_slot_free(S, nullptr);
-    int64_t ring = 0;
-
-    for (int64_t i = 0; i < nTm; ++i) {
-        for (int64_t jj = 0; jj < nTn; ++jj) {
-            const int64_t j = (i % 2 == 0) ? jj : nTn - 1 - jj;     // snake: reuse the last B panel
-            const int64_t t = i * nTn + jj;
-            hipStream_t s = h->streams[t % S];
-            const int64_t mi = std::min(tm, m - i * tm), nj = std::min(tn, n - j * tn);
-            const int64_t r0 = i * tm, c0 = j * tn;
-            // C tile location on the device
-            double *Ct;
-            int64_t ldCt;
-            if (kc == DEVICE || resident) {
-                Ct = dC + r0 + c0 * ldC;
-                ldCt = ldC;
-            } else {
-                Ct = static_cast<double *>(h->dC.ptr) + (size_t)(t % S) * tm * tn;
-                ldCt = tm;
-                // the slot's previous tile must have left the device before it is reused
-                if (c_slot_free[t % S]) {
-                    CA_HIP(hipStreamWaitEvent(h->h2d, c_slot_free[t % S], 0));
-                    CA_HIP(hipStreamWaitEvent(s, c_slot_free[t % S], 0));
-                }
-            }
-            TRACE("tile (%lld,%lld) stream %d C at %p ld %lld", (long long)i, (long long)j, (int)(t % S),
-                  (void *)Ct, (long long)ldCt);
-            if (kc != DEVICE && beta != 0.0) {
-                TRACE("  upload C tile %lldx%lld", (long long)mi, (long long)nj);
-                if ((rc = upload(h, Ct, ldCt, c + r0 + c0 * ldc, ldc, mi, nj, kc))) return rc;
-                hipEvent_t ev;
-                if ((rc = record(h, h->h2d, &ev))) return rc;
-                CA_HIP(hipStreamWaitEvent(s, ev, 0));
-            }
-            if (scale_only) {
-                TRACE("  scale by %g", beta);
-                if ((rc = launch_scale(mi, nj, beta, Ct, ldCt, s))) return rc;
-            }
-            for (int64_t p = 0; p < nTk; ++p) {
-                const int64_t k0 = p * tk, kp = std::min(tk, k - k0);
-                const double *Ap, *Bp;
-                int64_t ldAp = ldA, ldBp = ldB;
-                // ---- A block (rows r0..r0+mi of op(A), panel k0..k0+kp)
-                if (A.kind == DEVICE) {
-                    Ap = op_block(a, lda, ta, r0, k0);
-                } else if (resident) {
-                    hipEvent_t &e = evA[i * nTk + p];
-                    if (!e) {
-                        if ((rc = upload_op_block(h, A, dA, ldA, r0, k0, mi, kp, false))) return rc;
-                        if ((rc = record(h, h->h2d, &e))) return rc;
-                    }
-                    CA_HIP(hipStreamWaitEvent(s, e, 0));
-                    Ap = op_block(dA, ldA, ta, r0, k0);
-                } else {
-                    const int q = ring % 2;
-                    double *slot = static_cast<double *>(h->dA.ptr) + (size_t)q * tm * tk;
-                    ldAp = (ta == 'N') ? mi : kp;
-                    if (slot_free_A[q]) CA_HIP(hipStreamWaitEvent(h->h2d, slot_free_A[q], 0));
-                    if ((rc = upload_op_block(h, A, slot, ldAp, r0, k0, mi, kp, true))) return rc;
-                    hipEvent_t e;
-                    if ((rc = record(h, h->h2d, &e))) return rc;
-                    CA_HIP(hipStreamWaitEvent(s, e, 0));
-                    Ap = slot;
-                }
-                // ---- B block (panel k0..k0+kp, columns c0..c0+nj of op(B))
-                if (B.kind == DEVICE) {
-                    Bp = op_block(b, ldb, tb, k0, c0);
-                } else if (resident) {
-                    hipEvent_t &e = evB[p * nTn + j];
-                    if (!e) {
-                        if ((rc = upload_op_block(h, B, dB, ldB, k0, c0, kp, nj, false))) return rc;
-                        if ((rc = record(h, h->h2d, &e))) return rc;
-                    }
-                    CA_HIP(hipStreamWaitEvent(s, e, 0));
-                    Bp = op_block(dB, ldB, tb, k0, c0);
-                } else {
-                    const int q = ring % 2;
-                    double *slot = static_cast<double *>(h->dB.ptr) + (size_t)q * tk * tn;
-                    ldBp = (tb == 'N') ? kp : nj;
-                    if (slot_free_B[q]) CA_HIP(hipStreamWaitEvent(h->h2d, slot_free_B[q], 0));
-                    if ((rc = upload_op_block(h, B, slot, ldBp, k0, c0, kp, nj, true))) return rc;
-                    hipEvent_t e;
-                    if ((rc = record(h, h->h2d, &e))) return rc;
-                    CA_HIP(hipStreamWaitEvent(s, e, 0));
-                    Bp = slot;
-                }
-                rc = launch_dgemm(ta, tb, mi, nj, kp, alpha, Ap, ldAp, Bp, ldBp, p == 0 ? beta : 1.0, Ct, ldCt, s);
-                if (rc) return rc;
-                if (!resident) {
-                    const int q = ring % 2;
-                    hipEvent_t e;
-                    if ((rc = record(h, s, &e))) return rc;
-                    if (A.kind != DEVICE) slot_free_A[q] = e;
-                    if (B.kind != DEVICE) slot_free_B[q] = e;
-                    ++ring;
-                }
-            }
-            if (kc != DEVICE && copy_c_back) {
-                hipEvent_t done;
-                if ((rc = record(h, s, &done))) return rc;
-                CA_HIP(hipStreamWaitEvent(h->d2h, done, 0));
-                TRACE("  download C tile");
-                if ((rc = download(h, c + r0 + c0 * ldc, ldc, Ct, ldCt, mi, nj, kc))) return rc;
-                if (!resident) {
-                    hipEvent_t freed;
-                    if ((rc = record(h, h->d2h, &freed))) return rc;
-                    c_slot_free[t % S] = freed;
-                }
-            }
-        }
-    }
-    if ((rc = sync_all(h))) return rc;
-    if (kc != DEVICE && !copy_c_back) {
-        h->dev_c = dC;
-        h->dev_c_ld = ldC;
-    }
-    return COSMA_AMD_OK;
+    int rc = resident ? stream_resident(h, A, B, kc, m, n, k, alpha, beta, c, ldc, copy_c_back,
+                                        tm, tn, tk, nTm, nTn, nTk, defer_c, R)
+                      : stream_bounded(h, A, B, kc, m, n, k, alpha, beta, c, ldc,
+                                       tm, tn, tk, nTm, nTn, nTk);
+    if (rc) return rc;
+    return sync_all(h);
 }
 
 // ------------------------------------------------------ grid-layout multiply

@@ -65,6 +65,20 @@ def test_memory_kinds_resident(kinds, ta, tb):
     assert oracle.rel_fro(read_back(tC, m), ref) <= gemm_bound(k)
 
 
+@pytest.mark.parametrize("kinds", KINDS)
+@pytest.mark.parametrize("ta,tb,beta", [("T", "N", 0.7), ("N", "T", 1.0), ("N", "N", 0.0)])
+def test_resident_schedule_phases(kinds, ta, tb, beta):
+    """Three k-panels (one panel-major, two per tile), 3 x 3 uneven tiles (more
+    than the 4 deferred-C slots), and a first product split into two 256-deep
+    k-chunks; host C is added as beta * C after the last panel, device C takes
+    beta in the first product."""
+    h = make_handle((256, 192, 512))
+    m, n, k = 700, 450, 1300
+    tC, ref = run(h, ta, tb, m, n, k, -0.9, beta, kinds, seed=30, nan_c=(beta == 0.0))
+    assert h.last_mode() == 1
+    assert oracle.rel_fro(read_back(tC, m), ref) <= gemm_bound(k)
+
+
 @pytest.mark.parametrize("kinds", KINDS[:3])
 def test_bounded_ring_schedule(kinds):
     h = make_handle((128, 96, 64), streams=3)
