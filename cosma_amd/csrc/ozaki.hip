@@ -72,38 +72,88 @@ __global__ void clear_u64(unsigned long long *p, int64_t n) {
 // Statistics of op(X) seen as R vectors of length K (the rows of op(A), the
 // columns of op(B)): the max |x| of every vector (as bit patterns, for the
 // exponents) and, for every k, the sum of squares across the vectors (for the
-// error estimate).  One 64 x 64 tile per workgroup through LDS, loaded along
-// the contiguous dimension; one atomic per vector and per k per tile.
-// CONTIG: element (r, k) at x[k + r * ld]; otherwise at x[r + k * ld].
+// error estimate).  No LDS: a wave reads 64 consecutive elements along the
+// contiguous dimension per load, keeps the reduction along the other
+// dimension in registers, and reduces across its lanes with shuffles.
+// CONTIG (element (r, k) at x[k + r * ld]): lane = k, wave = 64 vectors, one
+// wave max per vector; otherwise (x[r + k * ld]): lane = r, wave = 64 k, one
+// wave sum per k.  Workgroups cover 64 lanes x STATS_SPAN of the other
+// dimension (4 waves x STATS_SPAN / 4).
+constexpr int STATS_SPAN = 256;
+
 template <bool CONTIG>
-__global__ void stats_kernel(const double *__restrict__ x, int64_t ld, int64_t R, int64_t K,
-                             unsigned long long *maxbits, double *ksq) {
-    __shared__ double tile[64][65];
-    const int64_t r0 = (int64_t)blockIdx.x * 64, k0 = (int64_t)blockIdx.y * 64;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int idx = threadIdx.x + 256 * q;
-        const int fast = idx & 63, slow = idx >> 6;
-        const int rr = CONTIG ? slow : fast, kk = CONTIG ? fast : slow;
-        const int64_t r = r0 + rr, k = k0 + kk;
-        double v = 0.0;
-        if (r < R && k < K) v = CONTIG ? x[k + r * ld] : x[r + k * ld];
-        tile[rr][kk] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x < 64) {
-        const int rr = threadIdx.x;
-        unsigned long long best = 0;
-        for (int kk = 0; kk < 64; ++kk) {
-            const unsigned long long b = __double_as_longlong(fabs(tile[rr][kk]));
-            best = b > best ? b : best;
-        }
-        if (r0 + rr < R) atomicMax(&maxbits[r0 + rr], best);
-    } else if (threadIdx.x < 128) {
-        const int kk = threadIdx.x - 64;
+__global__ __launch_bounds__(256) void stats_kernel(const double *__restrict__ x, int64_t ld, int64_t R, int64_t K,
+                                                    unsigned long long *maxbits, double *ksq) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int PER_WAVE = STATS_SPAN / 4, U = 8;
+    if (CONTIG) {
+        const int64_t k = (int64_t)blockIdx.x * 64 + lane;
+        const int64_t rb = (int64_t)blockIdx.y * STATS_SPAN + wave * PER_WAVE;
+        const bool kin = k < K;
         double ss = 0.0;
-        for (int rr = 0; rr < 64; ++rr) ss = fma(tile[rr][kk], tile[rr][kk], ss);
-        if (k0 + kk < K) atomicAdd(&ksq[k0 + kk], ss);
+        for (int q0 = 0; q0 < PER_WAVE; q0 += U) {
+            double v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int64_t r = rb + q0 + u;
+                v[u] = (kin && r < R) ? x[k + r * ld] : 0.0;
+            }
+            // U independent lane reductions, stepped together so their
+            // shuffle latencies overlap
+            unsigned long long b[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                ss = fma(v[u], v[u], ss);
+                b[u] = __double_as_longlong(fabs(v[u]));
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const unsigned long long o = __shfl_xor(b[u], off);
+                    b[u] = o > b[u] ? o : b[u];
+                }
+            }
+            if (lane < U && rb + q0 + lane < R) {
+                unsigned long long mine = b[0];
+#pragma unroll
+                for (int u = 1; u < U; ++u) mine = lane == u ? b[u] : mine;
+                atomicMax(&maxbits[rb + q0 + lane], mine);
+            }
+        }
+        if (kin) atomicAdd(&ksq[k], ss);
+    } else {
+        const int64_t r = (int64_t)blockIdx.x * 64 + lane;
+        const int64_t kb = (int64_t)blockIdx.y * STATS_SPAN + wave * PER_WAVE;
+        const bool rin = r < R;
+        unsigned long long best = 0;
+        for (int q0 = 0; q0 < PER_WAVE; q0 += U) {
+            double v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int64_t k = kb + q0 + u;
+                v[u] = (rin && k < K) ? x[r + k * ld] : 0.0;
+            }
+            double sq[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const unsigned long long b = __double_as_longlong(fabs(v[u]));
+                best = b > best ? b : best;
+                sq[u] = v[u] * v[u];
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) sq[u] += __shfl_xor(sq[u], off);
+            }
+            if (lane < U && kb + q0 + lane < K) {
+                double mine = sq[0];
+#pragma unroll
+                for (int u = 1; u < U; ++u) mine = lane == u ? sq[u] : mine;
+                atomicAdd(&ksq[kb + q0 + lane], mine);
+            }
+        }
+        if (rin) atomicMax(&maxbits[r], best);
     }
 }
 
@@ -153,18 +203,33 @@ __device__ __forceinline__ void split16(const SplitArgs &a, const double (&v)[16
         double iv[16];
 #pragma unroll
         for (int j = 0; j < 16; ++j) iv[j] = rint(ldexp(v[j], a.P - e));
-        for (int l = 0; l < a.count; ++l) {
-            const double m = (double)a.m[l], im = a.inv_m[l];
-            const int hi = (a.m[l] - 1) / 2, lo = -(a.m[l] / 2);
-            union { i32x4 q; int8_t b[16]; } u;
+        if (a.P <= 50) {
+            // |iv| <= 2^50: iv * (1/m) is within |iv| / m * 2^-52 < 1 / (2m) of
+            // iv / m, closer than iv / m ever lies to a half-integer it does
+            // not sit on, so q = rint(iv / m) exactly and r = iv - m q lies in
+            // [-m/2, m/2]; only m = 256 reaches +-128, and 128 wraps to -128,
+            // the same residue.
+            for (int l = 0; l < a.count; ++l) {
+                const double m = (double)a.m[l], im = a.inv_m[l];
+                union { i32x4 q; int8_t b[16]; } u;
 #pragma unroll
-            for (int j = 0; j < 16; ++j) {
-                int r = (int)fma(-m, rint(iv[j] * im), iv[j]);   // exact: integers below 2^53
-                if (r > hi) r -= a.m[l];
-                if (r < lo) r += a.m[l];
-                u.b[j] = (int8_t)r;
+                for (int j = 0; j < 16; ++j) u.b[j] = (int8_t)(int)fma(-m, rint(iv[j] * im), iv[j]);
+                *reinterpret_cast<i32x4 *>(dst + (int64_t)l * a.slice_stride) = u.q;
             }
-            *reinterpret_cast<i32x4 *>(dst + (int64_t)l * a.slice_stride) = u.q;
+        } else {
+            for (int l = 0; l < a.count; ++l) {
+                const double m = (double)a.m[l], im = a.inv_m[l];
+                const int hi = (a.m[l] - 1) / 2, lo = -(a.m[l] / 2);
+                union { i32x4 q; int8_t b[16]; } u;
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    int r = (int)fma(-m, rint(iv[j] * im), iv[j]);   // exact: FMA of integers
+                    if (r > hi) r -= a.m[l];
+                    if (r < lo) r += a.m[l];
+                    u.b[j] = (int8_t)r;
+                }
+                *reinterpret_cast<i32x4 *>(dst + (int64_t)l * a.slice_stride) = u.q;
+            }
         }
     } else {
         double r[16];
@@ -233,64 +298,84 @@ __global__ __launch_bounds__(256) void split_strided(SplitArgs a) {
 
 // ------------------------------------------------------ recombination
 // Each thread recombines 4 consecutive rows of one column (one 4-byte load
-// per residue plane; ldr is a multiple of 4).
-__global__ void crt_epilogue(CrtConst cc, const int8_t *__restrict__ res, int64_t ldr, int64_t res_stride,
-                             int64_t M, int64_t N, int P, const int *__restrict__ E, const int *__restrict__ F,
-                             double alpha, double beta, double *__restrict__ C, int64_t ldc) {
-    const int64_t quads = ldr / 4;
-    const int64_t total = quads * N;
-    for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
-         idx += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t j = idx / quads, i0 = (idx - j * quads) * 4;
-        if (i0 >= M) continue;
+// per residue plane, the plane loop unrolled by 4 so that several loads are
+// in flight; 8 rows per thread or no unrolling measured 1.3x-1.6x slower);
+// blockIdx.y strides over the columns.  All arithmetic is exact FP64 on integers below 2^53:
+//   limb sums    |s| <= 20 * 128 * 2^41 < 2^49.4,
+//   quotient     |q| <= 20 * 128 (S / M with |c_l| <= 128, W_l < M),
+//   q * limb     < 2^11.4 * 2^41 = 2^52.4, so t = s - q * limb is one exact
+//                FMA, |t| < 2^52.5, and carries below 2^11.5 keep the next
+//                limb below 2^53.
+constexpr int CRT_ROWS = 4;
+constexpr int64_t PLANE_PAD = 3 * 1024;   // bytes; a multiple of 16, not of a large power of two
+// NL limbs of 41 bits: 3 while M < 2^122 (up to 15 moduli), else 4.
+template <int NL>
+__global__ __launch_bounds__(256) void crt_epilogue(CrtConst cc, const int8_t *__restrict__ res, int64_t ldr,
+                                                    int64_t res_stride, int64_t M, int64_t N, int P,
+                                                    const int *__restrict__ E, const int *__restrict__ F,
+                                                    double alpha, double beta, double *__restrict__ C,
+                                                    int64_t ldc) {
+    const int64_t i0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * CRT_ROWS;
+    if (i0 >= M) return;
+    constexpr double TWO41 = 2199023255552.0, INV_TWO41 = 1.0 / 2199023255552.0;
+    double ml[NL], scale[NL];   // limbs of M; 2^(41 x)
+#pragma unroll
+    for (int x = 0; x < NL; ++x) {
+        ml[x] = (double)cc.mlimb[x];
+        scale[x] = x == 0 ? 1.0 : scale[x - 1] * TWO41;
+    }
+    for (int64_t j = blockIdx.y; j < N; j += gridDim.y) {
         const int8_t *rp = res + i0 + j * ldr;
-        // S = sum_l c_l W_l limb by limb: |c_l| <= 128 and limbs < 2^41, so
-        // every partial sum of at most 20 terms stays below 2^53 (exact FMAs)
-        double s[4][NLIMB];
+        // S = sum_l c_l W_l limb by limb
+        double s[CRT_ROWS][NL];
 #pragma unroll
-        for (int e = 0; e < 4; ++e)
+        for (int e = 0; e < CRT_ROWS; ++e)
 #pragma unroll
-            for (int q = 0; q < NLIMB; ++q) s[e][q] = 0.0;
+            for (int q = 0; q < NL; ++q) s[e][q] = 0.0;
+#pragma unroll 4
         for (int l = 0; l < cc.n; ++l) {
             const uint32_t word = *reinterpret_cast<const uint32_t *>(rp + (int64_t)l * res_stride);
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
+            for (int e = 0; e < CRT_ROWS; ++e) {
                 const double c = (double)(int8_t)(word >> (8 * e));
 #pragma unroll
-                for (int q = 0; q < NLIMB; ++q) s[e][q] = fma(c, cc.w[l][q], s[e][q]);
+                for (int q = 0; q < NL; ++q) s[e][q] = fma(c, cc.w[l][q], s[e][q]);
             }
         }
-        const double fj = (double)F[j];
+        const int fj = F[j];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
+        for (int e = 0; e < CRT_ROWS; ++e) {
             const int64_t i = i0 + e;
             if (i >= M) break;
             // q = round(S / M); |A'B'| < M / 2 by the choice of P, and sd * inv_M
             // is within ~2^-40 of S / M, so the rounding cannot fall on the
             // wrong side of a tie
-            const double sd = ldexp(s[e][3], 3 * LIMB) + ldexp(s[e][2], 2 * LIMB) + ldexp(s[e][1], LIMB) + s[e][0];
-            const int64_t q = (int64_t)rint(sd * cc.inv_M);
-            int64_t t[NLIMB];
+            double sd = s[e][NL - 1] * scale[NL - 1];
 #pragma unroll
-            for (int x = 0; x < NLIMB; ++x) t[x] = (int64_t)s[e][x] - q * cc.mlimb[x];
+            for (int x = NL - 2; x >= 0; --x) sd += s[e][x] * scale[x];
+            const double q = rint(sd * cc.inv_M);
+            double t[NL];
 #pragma unroll
-            for (int x = 0; x < NLIMB - 1; ++x) {    // normalise: limbs 0..2 in [0, 2^41)
-                const int64_t carry = t[x] >> LIMB;  // arithmetic shift: floor division
-                t[x] -= carry << LIMB;
+            for (int x = 0; x < NL; ++x) t[x] = fma(-q, ml[x], s[e][x]);
+#pragma unroll
+            for (int x = 0; x < NL - 1; ++x) {    // normalise: limbs below the top in [0, 2^41)
+                const double carry = floor(t[x] * INV_TWO41);
+                t[x] = fma(-carry, TWO41, t[x]);
                 t[x + 1] += carry;
             }
-            // value = t3 2^123 + t2 2^82 + t1 2^41 + t0, each term exact in FP64;
-            // compensated sum of the three largest, then the low limb
-            const double h3 = ldexp((double)t[3], 3 * LIMB), h2 = ldexp((double)t[2], 2 * LIMB);
-            const double h1 = ldexp((double)t[1], LIMB);
-            double sm = h3 + h2;
-            double bb = sm - h3;
-            double err = (h3 - (sm - bb)) + (h2 - bb);
-            const double sm2 = sm + h1;
-            bb = sm2 - sm;
-            err += (sm - (sm2 - bb)) + (h1 - bb);
-            const double v = sm2 + (err + (double)t[0]);
-            const double val = alpha * ldexp(v, E[i] + (int)fj - 2 * P);
+            // value = sum_x t_x 2^(41 x), each term exact in FP64; compensated
+            // sum from the top limb down to limb 1, then the low limb
+            double sm = t[NL - 1] * scale[NL - 1], err = 0.0;
+#pragma unroll
+            for (int x = NL - 2; x >= 1; --x) {
+                const double h = t[x] * scale[x];
+                const double sn = sm + h;
+                const double bb = sn - sm;
+                err += (sm - (sn - bb)) + (h - bb);
+                sm = sn;
+            }
+            const double v = sm + (err + t[0]);
+            const double val = alpha * ldexp(v, E[i] + fj - 2 * P);
             double *cp = C + i + j * ldc;
             *cp = (beta == 0.0) ? val : fma(beta, *cp, val);
         }
@@ -448,7 +533,7 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
         return COSMA_AMD_ERR_UNSUPPORTED;
     }
     const int64_t Mp = round_up(m, 256), Np = round_up(n, 256), Kp = round_up(k, 256);
-    const int64_t ldr = round_up(m, 4);                         // residue output ld
+    const int64_t ldr = round_up(m, 8);                         // residue output ld
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     auto grow = [](DevBuf &buf, size_t need) -> int {
         if (buf.bytes >= need) return COSMA_AMD_OK;
@@ -480,11 +565,15 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
     int *F = E + Mp;
     hipLaunchKernelGGL(clear_u64, dim3(grid_for(nclear)), dim3(256), 0, s, maxA, nclear);
     const bool a_rows_contig = ta == 'T', b_rows_contig = tb == 'N';   // op rows contiguous in k
-    const dim3 ga((unsigned)((m + 63) / 64), (unsigned)((k + 63) / 64)), gb((unsigned)((n + 63) / 64), (unsigned)((k + 63) / 64));
-    if (a_rows_contig) hipLaunchKernelGGL(stats_kernel<true>, ga, dim3(256), 0, s, a, lda, m, k, maxA, ksqA);
-    else hipLaunchKernelGGL(stats_kernel<false>, ga, dim3(256), 0, s, a, lda, m, k, maxA, ksqA);
-    if (b_rows_contig) hipLaunchKernelGGL(stats_kernel<true>, gb, dim3(256), 0, s, b, ldb, n, k, maxB, ksqB);
-    else hipLaunchKernelGGL(stats_kernel<false>, gb, dim3(256), 0, s, b, ldb, n, k, maxB, ksqB);
+    // grid: x over the contiguous dimension in 64s, y over the other in STATS_SPAN
+    auto stats_grid = [](int64_t R, int64_t K, bool contig) {
+        const int64_t fast = contig ? K : R, slow = contig ? R : K;
+        return dim3((unsigned)((fast + 63) / 64), (unsigned)((slow + STATS_SPAN - 1) / STATS_SPAN));
+    };
+    if (a_rows_contig) hipLaunchKernelGGL(stats_kernel<true>, stats_grid(m, k, true), dim3(256), 0, s, a, lda, m, k, maxA, ksqA);
+    else hipLaunchKernelGGL(stats_kernel<false>, stats_grid(m, k, false), dim3(256), 0, s, a, lda, m, k, maxA, ksqA);
+    if (b_rows_contig) hipLaunchKernelGGL(stats_kernel<true>, stats_grid(n, k, true), dim3(256), 0, s, b, ldb, n, k, maxB, ksqB);
+    else hipLaunchKernelGGL(stats_kernel<false>, stats_grid(n, k, false), dim3(256), 0, s, b, ldb, n, k, maxB, ksqB);
     hipLaunchKernelGGL(exponents_kernel, dim3(grid_for(m)), dim3(256), 0, s, maxA, m, slices, E, flag);
     hipLaunchKernelGGL(exponents_kernel, dim3(grid_for(n)), dim3(256), 0, s, maxB, n, slices, F, flag);
     CA_HIP(hipGetLastError());
@@ -542,8 +631,14 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
     }
 
     // ---- workspace: slices / residues of A and B, then the INT8 GEMM output
-    const size_t a_bytes = (size_t)count * Mp * Kp, b_bytes = (size_t)count * Np * Kp;
-    const size_t out_bytes = slices ? (size_t)ldr * n * 8 : (size_t)count * ldr * n;
+    // Planes (one per modulus / slice) are PLANE_PAD bytes longer than their
+    // data: at the usual power-of-two sizes the planes would otherwise start
+    // at addresses 2^26 apart, and the split stores and recombination loads,
+    // which touch the same offset of every plane together, would pile onto
+    // the same memory channels.
+    const int64_t a_plane = Mp * Kp + PLANE_PAD, b_plane = Np * Kp + PLANE_PAD, r_plane = ldr * n + PLANE_PAD;
+    const size_t a_bytes = (size_t)count * a_plane, b_bytes = (size_t)count * b_plane;
+    const size_t out_bytes = slices ? (size_t)ldr * n * 8 : (size_t)count * r_plane;
     if ((rc = grow(h->ozaki_ws, al(a_bytes) + al(b_bytes) + al(out_bytes)))) return rc;
     char *ws = static_cast<char *>(h->ozaki_ws.ptr);
     int8_t *As = reinterpret_cast<int8_t *>(ws);
@@ -562,20 +657,20 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
         else
             hipLaunchKernelGGL(split_strided, dim3((unsigned)(Rpad / 64), (unsigned)(Kp / 64)), dim3(256), 0, s, sa);
     };
-    split(a, lda, m, Mp, a_rows_contig, E, As, Mp * Kp);
-    split(b, ldb, n, Np, b_rows_contig, F, Bs, Np * Kp);
+    split(a, lda, m, Mp, a_rows_contig, E, As, a_plane);
+    split(b, ldb, n, Np, b_rows_contig, F, Bs, b_plane);
     CA_HIP(hipGetLastError());
 
     // ---- 3. INT8 products
     I8Problem pr{};
     pr.M = m; pr.N = n; pr.K = Kp;
-    pr.a = As; pr.lda = Kp; pr.a_stride = Mp * Kp;
-    pr.b = Bs; pr.ldb = Kp; pr.b_stride = Np * Kp;
+    pr.a = As; pr.lda = Kp; pr.a_stride = a_plane;
+    pr.b = Bs; pr.ldb = Kp; pr.b_stride = b_plane;
     pr.out = out; pr.ldo = ldr;
     I8Jobs jobs{};
     int64_t products = 0;
     if (!slices) {
-        pr.out_stride = ldr * n;
+        pr.out_stride = r_plane;
         for (int l = 0; l < count; ++l) {
             jobs.npairs[l] = 1;
             jobs.pa[l][0] = (unsigned char)l;
@@ -586,8 +681,14 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
         products = count;
         CrtConst cc;
         crt_constants(count, cc);
-        hipLaunchKernelGGL(crt_epilogue, dim3(grid_for(ldr / 4 * n)), dim3(256), 0, s, cc,
-                           static_cast<const int8_t *>(out), ldr, ldr * n, m, n, P, E, F, alpha, beta, c, ldc);
+        const dim3 gcrt((unsigned)((m + 256 * CRT_ROWS - 1) / (256 * CRT_ROWS)),
+                        (unsigned)std::min<int64_t>(n, 65535));
+        if (log2_moduli(count) < 3 * LIMB - 1)
+            hipLaunchKernelGGL(crt_epilogue<3>, gcrt, dim3(256), 0, s, cc, static_cast<const int8_t *>(out), ldr,
+                               r_plane, m, n, P, E, F, alpha, beta, c, ldc);
+        else
+            hipLaunchKernelGGL(crt_epilogue<NLIMB>, gcrt, dim3(256), 0, s, cc, static_cast<const int8_t *>(out),
+                               ldr, r_plane, m, n, P, E, F, alpha, beta, c, ldc);
     } else {
         pr.out_stride = 0;
         // weight d = t + u (slice indices from 1): pairs with t + u = d, d <= count + 1
