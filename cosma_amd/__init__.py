@@ -96,6 +96,15 @@ SIGNATURES = {
     "cosma_amd_grouped_info": (ctypes.c_int, [c_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                               ctypes.POINTER(c_i64)]),
     "cosma_amd_grouped_destroy": (ctypes.c_int, [c_vp]),
+    "cosma_amd_grouped_ozaki_create": (ctypes.c_int, [c_vp, c_vp, c_i64, ctypes.c_int, ctypes.c_double,
+                                                      ctypes.POINTER(c_vp)]),
+    "cosma_amd_grouped_ozaki_run": (ctypes.c_int, [c_vp, c_vp]),
+    "cosma_amd_grouped_ozaki_graph_run": (ctypes.c_int, [c_vp, c_vp]),
+    "cosma_amd_grouped_ozaki_tune": (ctypes.c_int, [c_vp, c_vp]),
+    "cosma_amd_grouped_ozaki_set_grid": (ctypes.c_int, [c_vp, ctypes.c_int]),
+    "cosma_amd_grouped_ozaki_info": (ctypes.c_int, [c_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),
+                                                    ctypes.POINTER(c_i64), ctypes.POINTER(ctypes.c_int)]),
+    "cosma_amd_grouped_ozaki_destroy": (ctypes.c_int, [c_vp]),
     "cosma_amd_host_alloc": (ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(c_vp)]),
     "cosma_amd_host_free": (ctypes.c_int, [c_vp]),
     "cosma_amd_dgemm_layout": (ctypes.c_int, [c_vp, c_char, c_char, c_i64, c_i64, c_i64, c_dbl,
@@ -247,6 +256,53 @@ class Grouped:
     def close(self):
         if self.ptr:
             lib().cosma_amd_grouped_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GroupedOzaki:
+    """A grouped split-precision (moduli method) plan over device-resident problems.
+
+    problems: iterable of (transa, transb, m, n, k, alpha, a_ptr, lda, b_ptr, ldb, beta, c_ptr, ldc);
+    moduli = 0 chooses the count from `tolerance` on the operands present now.
+    """
+
+    def __init__(self, handle, problems, moduli=0, tolerance=1e-14):
+        probs = list(problems)
+        arr = (GemmDesc * max(1, len(probs)))()
+        for i, (ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc) in enumerate(probs):
+            arr[i] = GemmDesc(_ch(ta), _ch(tb), m, n, k, alpha, a, lda, b, ldb, beta, c, ldc)
+        p = c_vp()
+        check(lib().cosma_amd_grouped_ozaki_create(handle.ptr, ctypes.cast(arr, c_vp), len(probs), moduli,
+                                                   tolerance, ctypes.byref(p)))
+        self.ptr = p
+
+    def run(self, stream=None):
+        check(lib().cosma_amd_grouped_ozaki_run(self.ptr, stream))
+
+    def graph_run(self, stream=None):
+        check(lib().cosma_amd_grouped_ozaki_graph_run(self.ptr, stream))
+
+    def tune(self, stream=None):
+        check(lib().cosma_amd_grouped_ozaki_tune(self.ptr, stream))
+
+    def set_grid(self, grid):
+        check(lib().cosma_amd_grouped_ozaki_set_grid(self.ptr, grid))
+
+    def info(self):
+        mod, est, tiles, grid = ctypes.c_int(), ctypes.c_double(), c_i64(), ctypes.c_int()
+        check(lib().cosma_amd_grouped_ozaki_info(self.ptr, ctypes.byref(mod), ctypes.byref(est), ctypes.byref(tiles),
+                                                 ctypes.byref(grid)))
+        return {"moduli": mod.value, "error_estimate": est.value, "tiles": tiles.value, "grid": grid.value}
+
+    def close(self):
+        if self.ptr:
+            lib().cosma_amd_grouped_ozaki_destroy(self.ptr)
             self.ptr = None
 
     def __del__(self):

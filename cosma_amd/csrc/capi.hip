@@ -277,6 +277,53 @@ int cosma_amd_grouped_destroy(cosma_amd_grouped_t plan) {
     return COSMA_AMD_OK;
 }
 
+int cosma_amd_grouped_ozaki_create(cosma_amd_handle_t h, const cosma_amd_gemm_desc *problems, int64_t count,
+                                   int moduli, double tolerance, cosma_amd_grouped_ozaki_t *plan) {
+    clear_error();
+    CA_CHECK(h != nullptr, "grouped_ozaki_create: null handle");
+    CA_CHECK(plan != nullptr, "grouped_ozaki_create: null plan pointer");
+    CA_HIP(hipSetDevice(h->device));
+    return grouped_ozaki_create(h, problems, count, moduli, tolerance, plan);
+}
+
+int cosma_amd_grouped_ozaki_run(cosma_amd_grouped_ozaki_t plan, void *stream) {
+    clear_error();
+    CA_CHECK(plan != nullptr, "grouped_ozaki_run: null plan");
+    return grouped_ozaki_run(plan, static_cast<hipStream_t>(stream));
+}
+
+int cosma_amd_grouped_ozaki_graph_run(cosma_amd_grouped_ozaki_t plan, void *stream) {
+    clear_error();
+    CA_CHECK(plan != nullptr, "grouped_ozaki_graph_run: null plan");
+    return grouped_ozaki_graph_run(plan, static_cast<hipStream_t>(stream));
+}
+
+int cosma_amd_grouped_ozaki_tune(cosma_amd_grouped_ozaki_t plan, void *stream) {
+    clear_error();
+    CA_CHECK(plan != nullptr, "grouped_ozaki_tune: null plan");
+    return grouped_ozaki_tune(plan, static_cast<hipStream_t>(stream));
+}
+
+int cosma_amd_grouped_ozaki_set_grid(cosma_amd_grouped_ozaki_t plan, int grid) {
+    clear_error();
+    CA_CHECK(plan != nullptr, "grouped_ozaki_set_grid: null plan");
+    return grouped_ozaki_set_grid(plan, grid);
+}
+
+int cosma_amd_grouped_ozaki_info(cosma_amd_grouped_ozaki_t plan, int *moduli, double *error_estimate,
+                                 int64_t *tiles, int *grid) {
+    clear_error();
+    CA_CHECK(plan != nullptr, "grouped_ozaki_info: null plan");
+    grouped_ozaki_info(plan, moduli, error_estimate, tiles, grid);
+    return COSMA_AMD_OK;
+}
+
+int cosma_amd_grouped_ozaki_destroy(cosma_amd_grouped_ozaki_t plan) {
+    clear_error();
+    grouped_ozaki_destroy(plan);
+    return COSMA_AMD_OK;
+}
+
 int cosma_amd_host_alloc(size_t bytes, void **ptr) {
     clear_error();
     CA_CHECK(ptr != nullptr, "host_alloc: null output pointer");

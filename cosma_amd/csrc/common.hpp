@@ -114,6 +114,14 @@ int grouped_problem_flags(char ta, char tb, int64_t m, int64_t n, const double *
 int launch_grouped(int cfg, const GroupedProblem *probs, int nprob, int64_t total_tiles, int grid, hipStream_t s);
 int grouped_create(cosma_amd_handle_t h, const cosma_amd_gemm_desc *d, int64_t count, int tile_config,
                    cosma_amd_grouped_t *out);
+int grouped_ozaki_create(cosma_amd_handle_t h, const cosma_amd_gemm_desc *d, int64_t count, int moduli,
+                         double tol, cosma_amd_grouped_ozaki_t *out);
+int grouped_ozaki_run(cosma_amd_grouped_ozaki_t p, hipStream_t s);
+int grouped_ozaki_graph_run(cosma_amd_grouped_ozaki_t p, hipStream_t s);
+int grouped_ozaki_tune(cosma_amd_grouped_ozaki_t p, hipStream_t s);
+int grouped_ozaki_set_grid(cosma_amd_grouped_ozaki_t p, int grid);
+void grouped_ozaki_info(cosma_amd_grouped_ozaki_t p, int *moduli, double *estimate, int64_t *tiles, int *grid);
+int grouped_ozaki_destroy(cosma_amd_grouped_ozaki_t p);
 int grouped_run(cosma_amd_grouped_t p, hipStream_t s);
 int grouped_graph_run(cosma_amd_grouped_t p, hipStream_t s);
 int grouped_tune(cosma_amd_grouped_t p, hipStream_t s);

@@ -1,0 +1,681 @@
+// Grouped split-precision (moduli method) DGEMM plans: many independent
+// device-resident problems C_p <- alpha_p op(A_p) op(B_p) + beta_p C_p, each
+// FP64-accurate on the INT8 matrix cores, run as four grouped launches
+// (exponents, split, INT8 products, recombination) that a plan can replay as
+// one HIP graph.
+//
+// No counterpart in the reference, whose local multiplies are single plain
+// GEMMs (src/cosma/local_multiply.cpp:299-381); this is the split-precision
+// form of the FP64 grouped plans in grouped.hip.
+//
+// The method is the one of ozaki.hip (see there): per-row / per-column
+// power-of-two exponents, integer inputs of P bits, symmetric residues modulo
+// N pairwise-coprime moduli <= 256, exact INT32 products, Chinese-remainder
+// recombination in exact FP64.  One moduli count N serves the whole plan (so
+// the graph is fixed); each problem gets its own P from its k.  N is either
+// given or chosen at plan creation from the tolerance and the operands present
+// then: the per-problem error estimate gamma_p 2^-P_p of ozaki.hip (from the
+// row / column maxima and the per-k sums of squares, computed on the device)
+// must meet the tolerance, doubled, for every problem.  Later runs reuse N on
+// whatever the operands hold; the exponents are recomputed on every run.
+//
+// Layout of the workspace (one allocation owned by the plan): per problem the
+// residues of op(A) (N planes of Mp x Kp bytes, row r of op(A) k-contiguous),
+// of op(B) (N planes of Np x Kp, column j k-contiguous), and of the products
+// (N planes of ldr x n), where Mp, Np, Kp are m, n, k rounded up to 64 and
+// ldr = m rounded up to 4; then the exponents E (Mp) and F (Np) and one
+// non-finite flag per problem.  A problem whose operands hold Inf or NaN is
+// computed by plain FP64 dot products in the recombination kernel, so IEEE
+// propagation is kept.
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <vector>
+
+#include "ozaki.hpp"
+
+namespace cosma_amd {
+
+struct OzProblem {
+    int64_t m, n, k;
+    double alpha, beta;
+    const double *a;
+    int64_t lda;
+    const double *b;
+    int64_t ldb;
+    double *c;
+    int64_t ldc;
+    int ta, tb;              // 1: the stored operand is transposed
+    int P;                   // bits of the integer inputs
+    int scale_only;          // k == 0 or alpha == 0: C <- beta C
+    int64_t Mp, Np, Kp;      // padded extents
+    int8_t *ares, *bres;     // residues of op(A) rows / op(B) columns
+    int8_t *ores;            // residues of the products
+    int64_t oplane;          // bytes between product planes
+    int64_t ldr;             // leading dimension of a product plane
+    int *E, *F;              // exponents of the rows of op(A) / columns of op(B)
+    int64_t tile_start;      // first INT8 tile of the problem in the product launch
+};
+
+struct GzModuli {
+    int count;
+    int m[MAX_MODULI];
+    double inv_m[MAX_MODULI];
+    float inv_mf[MAX_MODULI];
+};
+
+}  // namespace cosma_amd
+
+struct cosma_amd_grouped_ozaki_s {
+    cosma_amd_handle_t h = nullptr;
+    int nprob = 0;
+    int moduli = 0;
+    double estimate = 0.0;
+    int64_t total_tiles = 0;
+    int grid = 1;
+    cosma_amd::GzModuli md{};
+    cosma_amd::CrtConst cc{};
+    cosma_amd::OzProblem *dev = nullptr;   // descriptors, sorted by decreasing cost
+    int *flags = nullptr;                  // one non-finite flag per problem
+    int *expo = nullptr;                   // exponents E and F of every problem
+    void *ws = nullptr;                    // residues
+    hipStream_t capture = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace cosma_amd {
+namespace {
+
+constexpr int GZ_TILE = 64;   // INT8 product tile (rows x columns); k-step 64 bytes
+constexpr int GZ_ROW = 80;    // LDS row stride in bytes: 16 consecutive rows start in 16 different bank groups
+
+// Vector r of a column-major operand x seen as R vectors of length K: element
+// (r, l) at x[l + r * ld] when `contig` (the vector runs along the stored
+// column), else at x[r + l * ld].
+__device__ __forceinline__ double vec_at(const double *x, int64_t ld, bool contig, int64_t r, int64_t l) {
+    return contig ? x[l + r * ld] : x[r + l * ld];
+}
+
+// Exponents of the R vectors (rows of op(A) or columns of op(B)); entries
+// [R, Rp) get 0.  A strided operand is read with one thread per vector (a
+// wave reads 64 consecutive elements per load); a contiguous one with one
+// wave per vector and a lane reduction.
+__device__ void vector_exponents(const double *x, int64_t ld, bool contig, int64_t R, int64_t K, int64_t Rp,
+                                 int *Eout, int *bad) {
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    if (!contig) {
+        for (int64_t r = t; r < Rp; r += blockDim.x) {
+            unsigned long long best = 0;
+            if (r < R)
+                for (int64_t l = 0; l < K; ++l) {
+                    const unsigned long long b = __double_as_longlong(fabs(x[r + l * ld]));
+                    best = b > best ? b : best;
+                }
+            if (best >= 0x7ff0000000000000ull) atomicOr(bad, 1);
+            Eout[r] = r < R ? row_exponent(best, 0) : 0;
+        }
+    } else {
+        const int waves = blockDim.x >> 6;
+        for (int64_t r = wave; r < Rp; r += waves) {
+            unsigned long long best = 0;
+            if (r < R)
+                for (int64_t l = lane; l < K; l += 64) {
+                    const unsigned long long b = __double_as_longlong(fabs(x[l + r * ld]));
+                    best = b > best ? b : best;
+                }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const unsigned long long o = __shfl_xor(best, off);
+                best = o > best ? o : best;
+            }
+            if (lane == 0) {
+                if (best >= 0x7ff0000000000000ull) atomicOr(bad, 1);
+                Eout[r] = r < R ? row_exponent(best, 0) : 0;
+            }
+        }
+    }
+}
+
+// One workgroup per problem: exponents of the rows of op(A) and the columns
+// of op(B), and the problem's non-finite flag.
+__global__ __launch_bounds__(256) void gz_exponents(const OzProblem *probs, int *flags) {
+    const OzProblem &q = probs[blockIdx.x];
+    __shared__ int bad;
+    if (threadIdx.x == 0) bad = 0;
+    __syncthreads();
+    if (!q.scale_only) {
+        vector_exponents(q.a, q.lda, q.ta != 0, q.m, q.k, q.Mp, q.E, &bad);
+        vector_exponents(q.b, q.ldb, q.tb == 0, q.n, q.k, q.Np, q.F, &bad);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) flags[blockIdx.x] = bad;
+}
+
+// Plan creation only: the error factor gamma_p of each problem (ozaki.hip's
+// estimate: relative Frobenius error ~ gamma 2^-P), +inf when it cannot be
+// formed, 0 for a zero product.  Reads the exponents written by gz_exponents.
+__global__ __launch_bounds__(256) void gz_gamma(const OzProblem *probs, const int *flags, double *gamma) {
+    const OzProblem &q = probs[blockIdx.x];
+    __shared__ int emax[2];
+    __shared__ double part[5][256];
+    const int t = threadIdx.x;
+    if (q.scale_only || flags[blockIdx.x]) {
+        if (t == 0) gamma[blockIdx.x] = q.scale_only ? 0.0 : INFINITY;
+        return;
+    }
+    if (t < 2) emax[t] = INT_MIN;
+    __syncthreads();
+    int ea = INT_MIN, fb = INT_MIN;
+    for (int64_t i = t; i < q.m; i += blockDim.x) ea = max(ea, q.E[i]);
+    for (int64_t j = t; j < q.n; j += blockDim.x) fb = max(fb, q.F[j]);
+    atomicMax(&emax[0], ea);
+    atomicMax(&emax[1], fb);
+    __syncthreads();
+    ea = emax[0];
+    fb = emax[1];
+    if (ea == ZERO_EXPONENT || fb == ZERO_EXPONENT) {
+        if (t == 0) gamma[blockIdx.x] = 0.0;
+        return;
+    }
+    double SA = 0.0, SB = 0.0, NA = 0.0, NB = 0.0, D = 0.0;
+    for (int64_t i = t; i < q.m; i += blockDim.x) SA += ldexp(1.0, 2 * (q.E[i] - ea));
+    for (int64_t j = t; j < q.n; j += blockDim.x) SB += ldexp(1.0, 2 * (q.F[j] - fb));
+    for (int64_t l = t; l < q.k; l += blockDim.x) {
+        double sa = 0.0, sb = 0.0;
+        for (int64_t i = 0; i < q.m; ++i) {
+            const double v = ldexp(vec_at(q.a, q.lda, q.ta != 0, i, l), -ea);
+            sa = fma(v, v, sa);
+        }
+        for (int64_t j = 0; j < q.n; ++j) {
+            const double v = ldexp(vec_at(q.b, q.ldb, q.tb == 0, j, l), -fb);
+            sb = fma(v, v, sb);
+        }
+        NA += sa;
+        NB += sb;
+        D = fma(sa, sb, D);
+    }
+    part[0][t] = SA;
+    part[1][t] = SB;
+    part[2][t] = NA;
+    part[3][t] = NB;
+    part[4][t] = D;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w)
+            for (int x = 0; x < 5; ++x) part[x][t] += part[x][t + w];
+        __syncthreads();
+    }
+    if (t == 0) {
+        const double d = part[4][0];
+        const double g = sqrt((part[0][0] * part[3][0] + part[2][0] * part[1][0]) / (12.0 * d));
+        gamma[blockIdx.x] = (d > 0.0 && isfinite(g)) ? g : INFINITY;
+    }
+}
+
+// Residues of every problem's op(A) rows (blockIdx.y = 0) or op(B) columns
+// (blockIdx.y = 1), padding rows and k with zeros.  A thread takes 16
+// consecutive k of one vector; for a strided operand consecutive threads take
+// consecutive vectors, so each of the 16 loads is one run across the wave.
+__global__ __launch_bounds__(256) void gz_split(const OzProblem *probs, GzModuli md) {
+    const OzProblem &q = probs[blockIdx.x];
+    if (q.scale_only) return;
+    const bool opA = blockIdx.y == 0;
+    const double *x = opA ? q.a : q.b;
+    const int64_t ld = opA ? q.lda : q.ldb;
+    const bool contig = opA ? q.ta != 0 : q.tb == 0;
+    const int64_t R = opA ? q.m : q.n, Rp = opA ? q.Mp : q.Np;
+    const int *ex = opA ? q.E : q.F;
+    int8_t *dst = opA ? q.ares : q.bres;
+    const int64_t plane = Rp * q.Kp, nkc = q.Kp / 16, chunks = Rp * nkc;
+    for (int64_t c = threadIdx.x; c < chunks; c += blockDim.x) {
+        const int64_t r = contig ? c / nkc : c % Rp;
+        const int64_t k0 = 16 * (contig ? c % nkc : c / Rp);
+        double v[16];
+        const bool rin = r < R;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) v[j] = (rin && k0 + j < q.k) ? vec_at(x, ld, contig, r, k0 + j) : 0.0;
+        moduli_residues16(v, rin ? ex[r] : 0, q.P, md.count, md.m, md.inv_m, dst + r * q.Kp + k0, plane);
+    }
+}
+
+// The INT8 products of every problem and modulus: persistent workgroups walk
+// the 64 x 64 output tiles (problems sorted by decreasing cost), 2 x 2 waves of
+// 32 x 32 (2 x 2 v_mfma_i32_16x16x64_i8 each), k-step 64 double-buffered
+// through LDS, exact INT32 accumulation, reduced modulo the tile's modulus
+// into symmetric int8 residues.
+__global__ __launch_bounds__(256) void gz_products(const OzProblem *probs, int nprob, int64_t total, GzModuli md) {
+    __shared__ __attribute__((aligned(16))) int8_t sa[2][GZ_TILE * GZ_ROW];
+    __shared__ __attribute__((aligned(16))) int8_t sb[2][GZ_TILE * GZ_ROW];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm = wave & 1, wn = wave >> 1, g = lane >> 4, li = lane & 15;
+    const int lrow = t >> 2, lchunk = t & 3;   // this thread's 16 bytes of a 64 x 64-byte step
+    for (int64_t item = blockIdx.x; item < total; item += gridDim.x) {
+        int lo = 0, hi = nprob - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) / 2;
+            if (probs[mid].tile_start <= item) lo = mid;
+            else hi = mid - 1;
+        }
+        const OzProblem &q = probs[lo];
+        int64_t local = item - q.tile_start;
+        const int64_t tm = q.Mp / GZ_TILE, per = tm * (q.Np / GZ_TILE);
+        const int l = (int)(local / per);
+        local -= (int64_t)l * per;
+        const int64_t m0 = (local % tm) * GZ_TILE, n0 = (local / tm) * GZ_TILE;
+        const int8_t *A = q.ares + (int64_t)l * q.Mp * q.Kp + (m0 + lrow) * q.Kp + 16 * lchunk;
+        const int8_t *B = q.bres + (int64_t)l * q.Np * q.Kp + (n0 + lrow) * q.Kp + 16 * lchunk;
+        const int nsteps = (int)(q.Kp / 64);
+        i32x4 acc[2][2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = i32x4{0, 0, 0, 0};
+        __syncthreads();   // the previous tile's fragment reads are done
+        i32x4 ra = *reinterpret_cast<const i32x4 *>(A), rb = *reinterpret_cast<const i32x4 *>(B);
+        for (int s = 0; s < nsteps; ++s) {
+            const int st = s & 1;
+            *reinterpret_cast<i32x4 *>(sa[st] + lrow * GZ_ROW + 16 * lchunk) = ra;
+            *reinterpret_cast<i32x4 *>(sb[st] + lrow * GZ_ROW + 16 * lchunk) = rb;
+            __syncthreads();
+            if (s + 1 < nsteps) {
+                ra = *reinterpret_cast<const i32x4 *>(A + 64 * (s + 1));
+                rb = *reinterpret_cast<const i32x4 *>(B + 64 * (s + 1));
+            }
+            i32x4 fa[2], fb[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+                fa[i] = *reinterpret_cast<const i32x4 *>(sa[st] + (wm * 32 + 16 * i + li) * GZ_ROW + 16 * g);
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                fb[j] = *reinterpret_cast<const i32x4 *>(sb[st] + (wn * 32 + 16 * j + li) * GZ_ROW + 16 * g);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[i], fb[j], acc[i][j], 0, 0, 0);
+        }
+        // lane 16 g + li holds rows 4g .. 4g + 3 of column li of each 16 x 16 block
+        const int mod = md.m[l];
+        const float inv = md.inv_mf[l];
+        int8_t *O = q.ores + (int64_t)l * q.oplane;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int64_t row = m0 + wm * 32 + 16 * i + 4 * g, col = n0 + wn * 32 + 16 * j + li;
+                if (col >= q.n) continue;
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (row + e < q.m) O[row + e + col * q.ldr] = (int8_t)sym_mod(acc[i][j][e], mod, inv);
+            }
+    }
+}
+
+// Recombination and epilogue: workgroup row blockIdx.x = problem, a thread
+// takes 4 consecutive rows of one column.  Problems with k == 0 or alpha == 0
+// only scale C; problems flagged non-finite are computed by FP64 dot products.
+template <int NL>
+__global__ __launch_bounds__(256) void gz_recombine(const OzProblem *probs, const int *flags, CrtConst cc) {
+    const OzProblem &q = probs[blockIdx.x];
+    const int64_t quads = (q.m + 3) / 4, items = quads * q.n;
+    const bool plain = flags[blockIdx.x] != 0;
+    for (int64_t it = threadIdx.x + (int64_t)blockIdx.y * blockDim.x; it < items;
+         it += (int64_t)blockDim.x * gridDim.y) {
+        const int64_t j = it / quads, i0 = (it - j * quads) * 4;
+        double val[4] = {0.0, 0.0, 0.0, 0.0};
+        if (q.scale_only) {
+            // C <- beta C below
+        } else if (plain) {
+            for (int e = 0; e < 4 && i0 + e < q.m; ++e) {
+                double acc = 0.0;
+                for (int64_t l = 0; l < q.k; ++l)
+                    acc = fma(vec_at(q.a, q.lda, q.ta != 0, i0 + e, l), vec_at(q.b, q.ldb, q.tb == 0, j, l), acc);
+                val[e] = q.alpha * acc;
+            }
+        } else {
+            double s[4][NL];
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+#pragma unroll
+                for (int x = 0; x < NL; ++x) s[e][x] = 0.0;
+            const int8_t *rp = q.ores + i0 + j * q.ldr;
+#pragma unroll 4
+            for (int l = 0; l < cc.n; ++l) {
+                const uint32_t word = *reinterpret_cast<const uint32_t *>(rp + (int64_t)l * q.oplane);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const double c = (double)(int8_t)(word >> (8 * e));
+#pragma unroll
+                    for (int x = 0; x < NL; ++x) s[e][x] = fma(c, cc.w[l][x], s[e][x]);
+                }
+            }
+            const int fj = q.F[j];
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                if (i0 + e < q.m) val[e] = q.alpha * ldexp(crt_value<NL>(s[e], cc), q.E[i0 + e] + fj - 2 * q.P);
+        }
+        for (int e = 0; e < 4 && i0 + e < q.m; ++e) {
+            double *cp = q.c + i0 + e + j * q.ldc;
+            *cp = (q.beta == 0.0) ? val[e] : fma(q.beta, *cp, val[e]);
+        }
+    }
+}
+
+constexpr int GZ_RECOMBINE_Y = 8;
+
+void drop_graph(cosma_amd_grouped_ozaki_t p) {
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    p->exec = nullptr;
+    p->graph = nullptr;
+}
+
+bool on_device(const void *ptr) {
+    hipPointerAttribute_t attr;
+    if (hipPointerGetAttributes(&attr, ptr) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+inline int64_t up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+}  // namespace
+
+int grouped_ozaki_destroy(cosma_amd_grouped_ozaki_t p) {
+    if (!p) return COSMA_AMD_OK;
+    (void)hipSetDevice(p->h->device);
+    drop_graph(p);
+    if (p->capture) (void)hipStreamDestroy(p->capture);
+    if (p->dev) (void)hipFree(p->dev);
+    if (p->flags) (void)hipFree(p->flags);
+    if (p->expo) (void)hipFree(p->expo);
+    if (p->ws) (void)hipFree(p->ws);
+    delete p;
+    return COSMA_AMD_OK;
+}
+
+int grouped_ozaki_run(cosma_amd_grouped_ozaki_t p, hipStream_t s) {
+    if (p->nprob == 0) return COSMA_AMD_OK;
+    CA_HIP(hipSetDevice(p->h->device));
+    hipLaunchKernelGGL(gz_exponents, dim3((unsigned)p->nprob), dim3(256), 0, s, p->dev, p->flags);
+    hipLaunchKernelGGL(gz_split, dim3((unsigned)p->nprob, 2), dim3(256), 0, s, p->dev, p->md);
+    if (p->total_tiles > 0)
+        hipLaunchKernelGGL(gz_products, dim3((unsigned)p->grid), dim3(256), 0, s, p->dev, p->nprob, p->total_tiles,
+                           p->md);
+    const dim3 gr((unsigned)p->nprob, GZ_RECOMBINE_Y);
+    if (crt_limbs(p->moduli) == 3)
+        hipLaunchKernelGGL(gz_recombine<3>, gr, dim3(256), 0, s, p->dev, p->flags, p->cc);
+    else
+        hipLaunchKernelGGL(gz_recombine<NLIMB>, gr, dim3(256), 0, s, p->dev, p->flags, p->cc);
+    CA_HIP(hipGetLastError());
+    return COSMA_AMD_OK;
+}
+
+int grouped_ozaki_graph_run(cosma_amd_grouped_ozaki_t p, hipStream_t s) {
+    if (p->nprob == 0) return COSMA_AMD_OK;
+    CA_HIP(hipSetDevice(p->h->device));
+    if (!p->exec) {
+        if (!p->capture) CA_HIP(hipStreamCreateWithFlags(&p->capture, hipStreamNonBlocking));
+        CA_HIP(hipStreamBeginCapture(p->capture, hipStreamCaptureModeThreadLocal));
+        const int rc = grouped_ozaki_run(p, p->capture);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(p->capture, &g);
+        if (rc != COSMA_AMD_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        CA_HIP(e);
+        p->graph = g;
+        CA_HIP(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+    }
+    CA_HIP(hipGraphLaunch(p->exec, s));
+    return COSMA_AMD_OK;
+}
+
+int grouped_ozaki_tune(cosma_amd_grouped_ozaki_t p, hipStream_t s) {
+    if (p->nprob == 0 || p->total_tiles == 0) return COSMA_AMD_OK;
+    CA_HIP(hipSetDevice(p->h->device));
+    hipEvent_t a, b;
+    CA_HIP(hipEventCreate(&a));
+    CA_HIP(hipEventCreate(&b));
+    float best = -1.0f;
+    int best_grid = p->grid;
+    int rc = COSMA_AMD_OK;
+    const int64_t cu = p->h->n_cu;
+    for (int64_t g : {cu, 2 * cu, 4 * cu, 8 * cu, 16 * cu, p->total_tiles}) {
+        p->grid = (int)std::max<int64_t>(1, std::min<int64_t>(g, p->total_tiles));
+        if ((rc = grouped_ozaki_run(p, s))) break;   // warm-up
+        if (hipEventRecord(a, s) != hipSuccess) { rc = COSMA_AMD_ERR_HIP; break; }
+        for (int r = 0; r < 3 && rc == COSMA_AMD_OK; ++r) rc = grouped_ozaki_run(p, s);
+        if (rc) break;
+        if (hipEventRecord(b, s) != hipSuccess || hipEventSynchronize(b) != hipSuccess) {
+            rc = COSMA_AMD_ERR_HIP;
+            break;
+        }
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, a, b);
+        if (best < 0.0f || ms < best) {
+            best = ms;
+            best_grid = p->grid;
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    p->grid = best_grid;
+    drop_graph(p);
+    if (rc) set_error("grouped ozaki: tuning failed");
+    return rc;
+}
+
+int grouped_ozaki_create(cosma_amd_handle_t h, const cosma_amd_gemm_desc *d, int64_t count, int moduli,
+                         double tol, cosma_amd_grouped_ozaki_t *out) {
+    *out = nullptr;
+    if (count < 0 || count > 0x7fffffff || (count > 0 && !d)) {
+        set_error("grouped ozaki: invalid problem list");
+        return COSMA_AMD_ERR_INVALID_ARG;
+    }
+    if (moduli != 0 && (moduli < 2 || moduli > MAX_MODULI)) {
+        set_error("grouped ozaki: %d moduli requested (2..%d supported)", moduli, MAX_MODULI);
+        return COSMA_AMD_ERR_INVALID_ARG;
+    }
+    if (tol <= 0.0) tol = 1e-14;
+    CA_HIP(hipSetDevice(h->device));
+    std::vector<OzProblem> probs;
+    std::vector<double> cost;
+    for (int64_t i = 0; i < count; ++i) {
+        const cosma_amd_gemm_desc &q = d[i];
+        const char ta = norm_trans(q.transa), tb = norm_trans(q.transb);
+        if (!ta || !tb || q.m < 0 || q.n < 0 || q.k < 0) {
+            set_error("grouped ozaki: problem %lld has an invalid transpose or a negative dimension", (long long)i);
+            return COSMA_AMD_ERR_INVALID_ARG;
+        }
+        if (q.m == 0 || q.n == 0) continue;
+        const bool scale_only = (q.k == 0 || q.alpha == 0.0);
+        const int64_t ar = ta == 'N' ? q.m : q.k, br = tb == 'N' ? q.k : q.n;
+        if (q.ldc < std::max<int64_t>(1, q.m) || !q.c || !on_device(q.c) ||
+            (!scale_only && (q.lda < std::max<int64_t>(1, ar) || q.ldb < std::max<int64_t>(1, br) || !q.a || !q.b ||
+                             !on_device(q.a) || !on_device(q.b)))) {
+            set_error("grouped ozaki: problem %lld has a leading dimension too small or an operand that is not "
+                      "device memory", (long long)i);
+            return COSMA_AMD_ERR_INVALID_ARG;
+        }
+        if (!scale_only && (double)q.k * 16384.0 >= 2147483648.0) {
+            set_error("grouped ozaki: problem %lld: k = %lld too large for exact INT32 accumulation", (long long)i,
+                      (long long)q.k);
+            return COSMA_AMD_ERR_UNSUPPORTED;
+        }
+        OzProblem g{};
+        g.m = q.m;
+        g.n = q.n;
+        g.k = q.k;
+        g.alpha = q.alpha;
+        g.beta = q.beta;
+        g.a = q.a;
+        g.lda = q.lda;
+        g.b = q.b;
+        g.ldb = q.ldb;
+        g.c = q.c;
+        g.ldc = q.ldc;
+        g.ta = ta == 'T';
+        g.tb = tb == 'T';
+        g.scale_only = scale_only;
+        g.Mp = up(q.m, GZ_TILE);
+        g.Np = up(q.n, GZ_TILE);
+        g.Kp = scale_only ? 0 : up(q.k, 64);
+        g.ldr = up(q.m, 4);
+        probs.push_back(g);
+        cost.push_back(scale_only ? 0.0 : (double)q.m * q.n * q.k);
+    }
+    std::vector<int> order(probs.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+    std::vector<OzProblem> sorted;
+    sorted.reserve(probs.size());
+    for (int i : order) sorted.push_back(probs[i]);
+
+    auto p = new cosma_amd_grouped_ozaki_s();
+    p->h = h;
+    p->nprob = (int)sorted.size();
+    p->grid = 1;
+    if (p->nprob == 0) {
+        p->moduli = moduli ? moduli : 2;
+        *out = p;
+        return COSMA_AMD_OK;
+    }
+    auto fail = [&](int rc) {
+        grouped_ozaki_destroy(p);
+        return rc;
+    };
+    auto alloc = [&](void **ptr, size_t bytes) -> int {
+        const hipError_t e = hipMalloc(ptr, std::max<size_t>(bytes, 256));
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            *ptr = nullptr;
+            set_error("grouped ozaki: allocation of %zu bytes failed (%s)", bytes, hipGetErrorString(e));
+            return COSMA_AMD_ERR_ALLOC;
+        }
+        return COSMA_AMD_OK;
+    };
+    int rc;
+    const size_t ndesc = sorted.size() * sizeof(OzProblem);
+    if ((rc = alloc(reinterpret_cast<void **>(&p->dev), ndesc))) return fail(rc);
+    if ((rc = alloc(reinterpret_cast<void **>(&p->flags), sorted.size() * sizeof(int)))) return fail(rc);
+
+    // exponents first (their space does not depend on the moduli count)
+    int64_t nexp = 0;
+    for (const auto &g : sorted) nexp += g.Mp + g.Np;
+    if ((rc = alloc(reinterpret_cast<void **>(&p->expo), (size_t)nexp * sizeof(int)))) return fail(rc);
+    {
+        int64_t off = 0;
+        for (auto &g : sorted) {
+            g.E = p->expo + off;
+            off += g.Mp;
+            g.F = p->expo + off;
+            off += g.Np;
+        }
+    }
+    auto upload = [&]() -> int {
+        CA_HIP(hipMemcpy(p->dev, sorted.data(), ndesc, hipMemcpyHostToDevice));
+        return COSMA_AMD_OK;
+    };
+
+    // error factors of the operands present now: they choose the count when
+    // none is given, and give the plan's estimate either way
+    if ((rc = upload())) return fail(rc);
+    std::vector<double> gamma(sorted.size());
+    {
+        double *dgamma = nullptr;
+        if ((rc = alloc(reinterpret_cast<void **>(&dgamma), sorted.size() * sizeof(double)))) return fail(rc);
+        hipLaunchKernelGGL(gz_exponents, dim3((unsigned)p->nprob), dim3(256), 0, 0, p->dev, p->flags);
+        hipLaunchKernelGGL(gz_gamma, dim3((unsigned)p->nprob), dim3(256), 0, 0, p->dev, p->flags, dgamma);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpy(gamma.data(), dgamma, gamma.size() * sizeof(double), hipMemcpyDeviceToHost);
+        (void)hipFree(dgamma);
+        if (e != hipSuccess) {
+            set_error("grouped ozaki: error estimation failed (%s)", hipGetErrorString(e));
+            return fail(COSMA_AMD_ERR_HIP);
+        }
+    }
+    // the doubled estimate of the worst problem for n moduli
+    auto worst = [&](int n) {
+        double w = 0.0;
+        for (size_t i = 0; i < sorted.size(); ++i) {
+            if (sorted[i].scale_only || gamma[i] == 0.0) continue;
+            const int P = std::min(moduli_bits(n, sorted[i].k), MAX_P);
+            if (P < 1) return (double)INFINITY;
+            w = std::max(w, 2.0 * gamma[i] * std::ldexp(1.0, -P));
+        }
+        return w;
+    };
+    if (moduli == 0) {
+        moduli = 2;
+        while (moduli < MAX_MODULI && !(worst(moduli) <= tol)) ++moduli;
+    }
+    p->estimate = worst(moduli);
+    p->moduli = moduli;
+    for (auto &g : sorted) {
+        if (g.scale_only) continue;
+        g.P = std::min(moduli_bits(moduli, g.k), MAX_P);
+        if (g.P < 1) {
+            set_error("grouped ozaki: %d moduli leave no precision for k = %lld", moduli, (long long)g.k);
+            return fail(COSMA_AMD_ERR_UNSUPPORTED);
+        }
+    }
+
+    // residues: per problem A planes, B planes, product planes (16-byte aligned)
+    size_t bytes = 0;
+    std::vector<size_t> offs(sorted.size());
+    for (size_t i = 0; i < sorted.size(); ++i) {
+        const auto &g = sorted[i];
+        offs[i] = bytes;
+        if (g.scale_only) continue;
+        bytes += (size_t)moduli * (g.Mp + g.Np) * g.Kp + (size_t)moduli * up(g.ldr * g.n, 16);
+    }
+    if ((rc = alloc(&p->ws, bytes))) return fail(rc);
+    int64_t tiles = 0;
+    for (size_t i = 0; i < sorted.size(); ++i) {
+        auto &g = sorted[i];
+        g.tile_start = tiles;
+        if (g.scale_only) continue;
+        int8_t *base = static_cast<int8_t *>(p->ws) + offs[i];
+        g.ares = base;
+        g.bres = base + (size_t)moduli * g.Mp * g.Kp;
+        g.ores = g.bres + (size_t)moduli * g.Np * g.Kp;
+        g.oplane = up(g.ldr * g.n, 16);
+        tiles += (g.Mp / GZ_TILE) * (g.Np / GZ_TILE) * moduli;
+    }
+    p->total_tiles = tiles;
+    if ((rc = upload())) return fail(rc);
+    p->md.count = moduli;
+    for (int l = 0; l < MAX_MODULI; ++l) {
+        p->md.m[l] = kModuli[l];
+        p->md.inv_m[l] = 1.0 / kModuli[l];
+        p->md.inv_mf[l] = 1.0f / (float)kModuli[l];
+    }
+    crt_constants(moduli, p->cc);
+    p->grid = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, (int64_t)h->n_cu * 8));
+    *out = p;
+    return COSMA_AMD_OK;
+}
+
+void grouped_ozaki_info(cosma_amd_grouped_ozaki_t p, int *moduli, double *estimate, int64_t *tiles, int *grid) {
+    if (moduli) *moduli = p->moduli;
+    if (estimate) *estimate = p->estimate;
+    if (tiles) *tiles = p->total_tiles;
+    if (grid) *grid = p->grid;
+}
+
+int grouped_ozaki_set_grid(cosma_amd_grouped_ozaki_t p, int grid) {
+    if (grid < 1) {
+        set_error("grouped ozaki: grid %d must be positive", grid);
+        return COSMA_AMD_ERR_INVALID_ARG;
+    }
+    p->grid = grid;
+    drop_graph(p);
+    return COSMA_AMD_OK;
+}
+
+}  // namespace cosma_amd

@@ -15,17 +15,6 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t T) {
     return (xcd < r) ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
 }
 
-// Symmetric residue of x modulo m, in [-(m/2), (m-1)/2].  The quotient comes
-// from an FP32 estimate: for |x| < 2^31 and m >= 2 the estimate is within one
-// of the rounded quotient, so one correction on each side is exact.
-__device__ __forceinline__ int sym_mod(int x, int m, float inv_m) {
-    const int q = (int)rintf((float)x * inv_m);
-    int r = x - q * m;
-    r -= (r > (m - 1) / 2) ? m : 0;
-    r += (r < -(m / 2)) ? m : 0;
-    return r;
-}
-
 template <int EPI>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void i8gemm_kernel(I8Problem p, I8Jobs jobs) {
     // LDS: two stages of 256 x 128-byte rows per operand (128 KB in all).

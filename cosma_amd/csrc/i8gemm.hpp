@@ -64,4 +64,16 @@ struct I8Problem {
 // split kernels pad with zeros), so tile loads never need bounds checks.
 int launch_i8gemm(const I8Problem &p, const I8Jobs &jobs, int Z, I8Epilogue epi, hipStream_t s);
 
+// Symmetric residue of x modulo m, in [-(m/2), (m-1)/2].  The quotient comes
+// from an FP32 estimate: for |x| < 2^31 and m >= 2 the estimate is within one
+// of the rounded quotient, so one correction on each side is exact.
+__device__ __forceinline__ int sym_mod(int x, int m, float inv_m) {
+    const int q = (int)rintf((float)x * inv_m);
+    int r = x - q * m;
+    r -= (r > (m - 1) / 2) ? m : 0;
+    r += (r < -(m / 2)) ? m : 0;
+    return r;
+}
+
+
 }  // namespace cosma_amd

@@ -31,32 +31,9 @@
 #include <cmath>
 #include <vector>
 
-#include "i8gemm.hpp"
+#include "ozaki.hpp"
 
 namespace cosma_amd {
-
-// pairwise-coprime moduli, largest first (256 = 2^8; 255 = 3*5*17; 253 = 11*23;
-// 247 = 13*19; 217 = 7*31; the rest are primes)
-static const int kModuli[20] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227,
-                                223, 217, 211, 199, 197, 193, 191, 181, 179, 173};
-constexpr int MAX_MODULI = 20;
-// CRT limbs: 4 limbs of 41 bits cover M < 2^164 (20 moduli: M ~ 2^155.4), and a
-// limb sum of 20 terms |c_l| * w < 20 * 2^7 * 2^41 < 2^53 stays exact in FP64.
-constexpr int LIMB = 41;
-constexpr int NLIMB = 4;
-// Largest P (bits of the integer inputs) for which the residues stay exact:
-// a' / m_l < 2^51 for every modulus (>= 173) keeps the split's FP64 quotient
-// within one of the true one, so a single correction step is exact.
-constexpr int MAX_P = 58;
-
-struct CrtConst {
-    int n;
-    int m[MAX_MODULI];
-    double inv_m[MAX_MODULI];
-    double w[MAX_MODULI][NLIMB];   // CRT weights W_l in 41-bit limbs (exact in FP64)
-    int64_t mlimb[NLIMB];          // M in 41-bit limbs
-    double inv_M;                  // 1 / M
-};
 
 namespace {
 
@@ -157,21 +134,6 @@ __global__ __launch_bounds__(256) void stats_kernel(const double *__restrict__ x
     }
 }
 
-// Exponent given to an all-zero row or column: its entries and products are
-// zero whatever the scaling, and it drops out of the error estimate.
-constexpr int ZERO_EXPONENT = -30000;
-
-// Row exponent from the max: max < 2^E.  For slices the leading digit must
-// stay within [-127, 127]: rows whose scaled max would round to 128 take E + 1.
-__device__ __forceinline__ int row_exponent(unsigned long long bits, int slices) {
-    const double mx = __longlong_as_double((long long)bits);
-    if (mx == 0.0) return ZERO_EXPONENT;
-    int e;
-    const double f = frexp(mx, &e);   // mx = f * 2^e, f in [0.5, 1)
-    if (slices && f * 128.0 >= 127.5) ++e;
-    return e;
-}
-
 __global__ void exponents_kernel(const unsigned long long *maxbits, int64_t R, int slices, int *E,
                                  int *nonfinite) {
     for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < R; r += (int64_t)gridDim.x * blockDim.x) {
@@ -200,37 +162,7 @@ struct SplitArgs {
 // 16 consecutive k of one row -> 16 bytes per modulus / slice
 __device__ __forceinline__ void split16(const SplitArgs &a, const double (&v)[16], int e, int8_t *dst) {
     if (a.method == 0) {
-        double iv[16];
-#pragma unroll
-        for (int j = 0; j < 16; ++j) iv[j] = rint(ldexp(v[j], a.P - e));
-        if (a.P <= 50) {
-            // |iv| <= 2^50: iv * (1/m) is within |iv| / m * 2^-52 < 1 / (2m) of
-            // iv / m, closer than iv / m ever lies to a half-integer it does
-            // not sit on, so q = rint(iv / m) exactly and r = iv - m q lies in
-            // [-m/2, m/2]; only m = 256 reaches +-128, and 128 wraps to -128,
-            // the same residue.
-            for (int l = 0; l < a.count; ++l) {
-                const double m = (double)a.m[l], im = a.inv_m[l];
-                union { i32x4 q; int8_t b[16]; } u;
-#pragma unroll
-                for (int j = 0; j < 16; ++j) u.b[j] = (int8_t)(int)fma(-m, rint(iv[j] * im), iv[j]);
-                *reinterpret_cast<i32x4 *>(dst + (int64_t)l * a.slice_stride) = u.q;
-            }
-        } else {
-            for (int l = 0; l < a.count; ++l) {
-                const double m = (double)a.m[l], im = a.inv_m[l];
-                const int hi = (a.m[l] - 1) / 2, lo = -(a.m[l] / 2);
-                union { i32x4 q; int8_t b[16]; } u;
-#pragma unroll
-                for (int j = 0; j < 16; ++j) {
-                    int r = (int)fma(-m, rint(iv[j] * im), iv[j]);   // exact: FMA of integers
-                    if (r > hi) r -= a.m[l];
-                    if (r < lo) r += a.m[l];
-                    u.b[j] = (int8_t)r;
-                }
-                *reinterpret_cast<i32x4 *>(dst + (int64_t)l * a.slice_stride) = u.q;
-            }
-        }
+        moduli_residues16(v, e, a.P, a.count, a.m, a.inv_m, dst, a.slice_stride);
     } else {
         double r[16];
 #pragma unroll
@@ -300,12 +232,8 @@ __global__ __launch_bounds__(256) void split_strided(SplitArgs a) {
 // Each thread recombines 4 consecutive rows of one column (one 4-byte load
 // per residue plane, the plane loop unrolled by 4 so that several loads are
 // in flight; 8 rows per thread or no unrolling measured 1.3x-1.6x slower);
-// blockIdx.y strides over the columns.  All arithmetic is exact FP64 on integers below 2^53:
-//   limb sums    |s| <= 20 * 128 * 2^41 < 2^49.4,
-//   quotient     |q| <= 20 * 128 (S / M with |c_l| <= 128, W_l < M),
-//   q * limb     < 2^11.4 * 2^41 = 2^52.4, so t = s - q * limb is one exact
-//                FMA, |t| < 2^52.5, and carries below 2^11.5 keep the next
-//                limb below 2^53.
+// blockIdx.y strides over the columns.  The limb sums and crt_value are exact
+// FP64 integer arithmetic (see crt_value).
 constexpr int CRT_ROWS = 4;
 constexpr int64_t PLANE_PAD = 3 * 1024;   // bytes; a multiple of 16, not of a large power of two
 // NL limbs of 41 bits: 3 while M < 2^122 (up to 15 moduli), else 4.
@@ -317,13 +245,6 @@ __global__ __launch_bounds__(256) void crt_epilogue(CrtConst cc, const int8_t *_
                                                     int64_t ldc) {
     const int64_t i0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * CRT_ROWS;
     if (i0 >= M) return;
-    constexpr double TWO41 = 2199023255552.0, INV_TWO41 = 1.0 / 2199023255552.0;
-    double ml[NL], scale[NL];   // limbs of M; 2^(41 x)
-#pragma unroll
-    for (int x = 0; x < NL; ++x) {
-        ml[x] = (double)cc.mlimb[x];
-        scale[x] = x == 0 ? 1.0 : scale[x - 1] * TWO41;
-    }
     for (int64_t j = blockIdx.y; j < N; j += gridDim.y) {
         const int8_t *rp = res + i0 + j * ldr;
         // S = sum_l c_l W_l limb by limb
@@ -347,34 +268,7 @@ __global__ __launch_bounds__(256) void crt_epilogue(CrtConst cc, const int8_t *_
         for (int e = 0; e < CRT_ROWS; ++e) {
             const int64_t i = i0 + e;
             if (i >= M) break;
-            // q = round(S / M); |A'B'| < M / 2 by the choice of P, and sd * inv_M
-            // is within ~2^-40 of S / M, so the rounding cannot fall on the
-            // wrong side of a tie
-            double sd = s[e][NL - 1] * scale[NL - 1];
-#pragma unroll
-            for (int x = NL - 2; x >= 0; --x) sd += s[e][x] * scale[x];
-            const double q = rint(sd * cc.inv_M);
-            double t[NL];
-#pragma unroll
-            for (int x = 0; x < NL; ++x) t[x] = fma(-q, ml[x], s[e][x]);
-#pragma unroll
-            for (int x = 0; x < NL - 1; ++x) {    // normalise: limbs below the top in [0, 2^41)
-                const double carry = floor(t[x] * INV_TWO41);
-                t[x] = fma(-carry, TWO41, t[x]);
-                t[x + 1] += carry;
-            }
-            // value = sum_x t_x 2^(41 x), each term exact in FP64; compensated
-            // sum from the top limb down to limb 1, then the low limb
-            double sm = t[NL - 1] * scale[NL - 1], err = 0.0;
-#pragma unroll
-            for (int x = NL - 2; x >= 1; --x) {
-                const double h = t[x] * scale[x];
-                const double sn = sm + h;
-                const double bb = sn - sm;
-                err += (sm - (sn - bb)) + (h - bb);
-                sm = sn;
-            }
-            const double v = sm + (err + t[0]);
+            const double v = crt_value<NL>(s[e], cc);
             const double val = alpha * ldexp(v, E[i] + fj - 2 * P);
             double *cp = C + i + j * ldc;
             *cp = (beta == 0.0) ? val : fma(beta, *cp, val);
@@ -463,7 +357,7 @@ double to_double(const U256 &x) {
 }
 }  // namespace
 
-static void crt_constants(int n, CrtConst &cc) {
+void crt_constants(int n, CrtConst &cc) {
     cc.n = n;
     U256 M;
     M.w[0] = 1;
@@ -683,7 +577,7 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
         crt_constants(count, cc);
         const dim3 gcrt((unsigned)((m + 256 * CRT_ROWS - 1) / (256 * CRT_ROWS)),
                         (unsigned)std::min<int64_t>(n, 65535));
-        if (log2_moduli(count) < 3 * LIMB - 1)
+        if (crt_limbs(count) == 3)
             hipLaunchKernelGGL(crt_epilogue<3>, gcrt, dim3(256), 0, s, cc, static_cast<const int8_t *>(out), ldr,
                                r_plane, m, n, P, E, F, alpha, beta, c, ldc);
         else

@@ -244,6 +244,42 @@ int cosma_amd_grouped_set_config(cosma_amd_grouped_t plan, int tile_config, int 
 int cosma_amd_grouped_info(cosma_amd_grouped_t plan, int *tile_config, int *grid, int64_t *tiles);
 int cosma_amd_grouped_destroy(cosma_amd_grouped_t plan);
 
+/* ------------------------------------------- grouped split-precision GEMM */
+typedef struct cosma_amd_grouped_ozaki_s *cosma_amd_grouped_ozaki_t;
+
+/*
+ * Plans the split-precision (moduli method, as cosma_amd_dgemm_ozaki) form of
+ * a grouped GEMM over `count` independent device-resident problems: per run,
+ * four grouped launches (exponents, residues, the INT8 products of every
+ * problem and modulus in one persistent launch over 64 x 64 tiles,
+ * recombination with the alpha / beta epilogue).  One moduli count serves the
+ * plan: `moduli` (2..20), or 0 to choose the smallest count whose doubled
+ * error estimate meets `tolerance` (<= 0: 1e-14) for every problem, estimated
+ * on the operands present at creation.  Problems whose operands hold Inf or
+ * NaN are computed by FP64 dot products.  The descriptors and a workspace
+ * (residues, exponents) are allocated at creation; runs allocate nothing and
+ * do not synchronise, so they can be captured.
+ * cosma_amd_grouped_ozaki_graph_run: captures a run into a hipGraph on first
+ * use and launches the graph on `stream`.
+ * cosma_amd_grouped_ozaki_tune: times the run with persistent product grids
+ * of 1, 2, 4, 8 and 16 workgroups per CU and one per tile (overwriting the
+ * problems' C) and keeps the fastest.
+ * cosma_amd_grouped_ozaki_info: moduli count, doubled error estimate of the
+ * worst problem at creation, number of INT8 tiles, product grid.
+ * No counterpart in the reference (its local multiplies are plain GEMMs,
+ * src/cosma/local_multiply.cpp:299-381).
+ */
+int cosma_amd_grouped_ozaki_create(cosma_amd_handle_t handle, const cosma_amd_gemm_desc *problems,
+                                   int64_t count, int moduli, double tolerance,
+                                   cosma_amd_grouped_ozaki_t *plan);
+int cosma_amd_grouped_ozaki_run(cosma_amd_grouped_ozaki_t plan, void *stream);
+int cosma_amd_grouped_ozaki_graph_run(cosma_amd_grouped_ozaki_t plan, void *stream);
+int cosma_amd_grouped_ozaki_tune(cosma_amd_grouped_ozaki_t plan, void *stream);
+int cosma_amd_grouped_ozaki_set_grid(cosma_amd_grouped_ozaki_t plan, int grid);
+int cosma_amd_grouped_ozaki_info(cosma_amd_grouped_ozaki_t plan, int *moduli, double *error_estimate,
+                                 int64_t *tiles, int *grid);
+int cosma_amd_grouped_ozaki_destroy(cosma_amd_grouped_ozaki_t plan);
+
 /* ------------------------------------------------------ pinned host memory */
 /*
  * Allocates `bytes` of page-locked host memory (hipHostMalloc) that the

@@ -1,0 +1,212 @@
+"""Grouped split-precision (moduli method) plans through the C ABI.
+
+With the same moduli count, each problem of a plan must come out
+bit-identical to cosma_amd_dgemm_ozaki on it (same exponents, same P, exact
+residue products and recombination), for the direct run and for every replay
+of the captured hipGraph.  Problems are also checked against the
+double-double oracle at the default tolerance, and the edge cases of the
+single-problem path are covered: k == 0, alpha == 0, beta == 0 with NaN in C,
+zero rows, and Inf / NaN operands (computed by FP64 dot products)."""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import from_dev, stored_shape, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def make_problems(count, seed, max_dim=200, phi=0.0):
+    import torch
+    import cosma_amd as ca
+    rng = np.random.default_rng(seed)
+    probs = []
+    for i in range(count):
+        m, n, k = (int(x) for x in rng.integers(1, max_dim + 1, 3))
+        if i % 17 == 3:
+            k = 0
+        ta, tb = "NT"[int(rng.integers(0, 2))], "NT"[int(rng.integers(0, 2))]
+        alpha = float(rng.choice([1.0, -1.3, 0.0 if i % 23 == 7 else 0.5]))
+        beta = float(rng.choice([0.0, 0.5, 1.0]))
+        A = oracle.uniform(*stored_shape(ta, m, max(k, 1)), 1000 + i)
+        B = oracle.uniform(*stored_shape(tb, max(k, 1), n), 2000 + i)
+        if phi:
+            A = np.asfortranarray(A * np.exp(phi * rng.standard_normal(A.shape)))
+            B = np.asfortranarray(B * np.exp(phi * rng.standard_normal(B.shape)))
+        if i % 13 == 2:
+            A[:] = 0.0                       # zero operand: zero product
+        C0 = oracle.uniform(m, n, 3000 + i)
+        pad = int(rng.integers(0, 2))
+        dA, lda = to_dev(A, A.shape[0] + pad)
+        dB, ldb = to_dev(B, B.shape[0] + pad)
+        dC, ldc = to_dev(C0)
+        if beta == 0.0:
+            dC.fill_(float("nan"))
+        probs.append(dict(ta=ta, tb=tb, m=m, n=n, k=k, alpha=alpha, beta=beta, A=A, B=B, C0=C0,
+                          dA=dA, lda=lda, dB=dB, ldb=ldb, dC=dC, ldc=ldc))
+    torch.cuda.synchronize()
+    return probs
+
+
+def descs(probs):
+    return [(p["ta"], p["tb"], p["m"], p["n"], p["k"], p["alpha"], p["dA"].data_ptr(), p["lda"],
+             p["dB"].data_ptr(), p["ldb"], p["beta"], p["dC"].data_ptr(), p["ldc"]) for p in probs]
+
+
+def reset_c(probs):
+    for p in probs:
+        src, _ = to_dev(p["C0"])
+        p["dC"].copy_(src)
+        if p["beta"] == 0.0:
+            p["dC"].fill_(float("nan"))
+
+
+def results(probs):
+    return [from_dev(p["dC"], p["m"]) for p in probs]
+
+
+def single_results(handle, probs, moduli):
+    """The same problems through cosma_amd_dgemm_ozaki with the same count."""
+    import torch
+    import cosma_amd as ca
+    out = []
+    for p in probs:
+        c = p["dC"].clone()
+        handle.dgemm_ozaki(p["ta"], p["tb"], p["m"], p["n"], p["k"], p["alpha"], p["dA"].data_ptr(), p["lda"],
+                           p["dB"].data_ptr(), p["ldb"], p["beta"], c.data_ptr(), p["ldc"],
+                           method=ca.OZAKI_MODULI, count=moduli)
+        out.append(c)
+    torch.cuda.synchronize()
+    return [from_dev(c, p["m"]) for c, p in zip(out, probs)]
+
+
+def same(got, exp):
+    return all(np.array_equal(g, e, equal_nan=True) for g, e in zip(got, exp))
+
+
+def oracle_result(p):
+    m, n, k = p["m"], p["n"], p["k"]
+    ref = p["C0"].copy(order="F")
+    oracle.dgemm(p["ta"], p["tb"], m, n, k, p["alpha"], p["A"], p["A"].shape[0], p["B"], p["B"].shape[0],
+                 p["beta"], ref, m, dd=True)
+    return ref
+
+
+@pytest.fixture(scope="module")
+def problems():
+    return make_problems(48, 5)
+
+
+@pytest.mark.parametrize("moduli", [8, 14, 17])
+def test_bit_identical_to_single_calls(handle, problems, moduli):
+    import torch
+    import cosma_amd as ca
+    reset_c(problems)
+    expected = single_results(handle, problems, moduli)
+    plan = ca.GroupedOzaki(handle, descs(problems), moduli=moduli)
+    assert plan.info()["moduli"] == moduli
+    plan.run()
+    torch.cuda.synchronize()
+    assert same(results(problems), expected)
+    plan.close()
+
+
+def test_default_tolerance_against_oracle(handle):
+    import torch
+    import cosma_amd as ca
+    probs = make_problems(40, 9, max_dim=300, phi=1.0)
+    reset_c(probs)
+    plan = ca.GroupedOzaki(handle, descs(probs))
+    info = plan.info()
+    assert 2 <= info["moduli"] <= 20 and info["error_estimate"] <= 1e-14
+    plan.run()
+    torch.cuda.synchronize()
+    for p, got in zip(probs, results(probs)):
+        ref = oracle_result(p)
+        if p["k"] == 0 or p["alpha"] == 0.0 or not p["A"].any():
+            exp = p["beta"] * p["C0"] if p["beta"] != 0 else np.zeros_like(ref)
+            assert np.array_equal(got, exp), (p["m"], p["n"], p["k"])
+        else:
+            assert oracle.rel_fro(got, ref) <= 2e-14, (p["m"], p["n"], p["k"], oracle.rel_fro(got, ref))
+    plan.close()
+
+
+def test_graph_replays_match_eager(handle, problems):
+    import torch
+    import cosma_amd as ca
+    reset_c(problems)
+    plan = ca.GroupedOzaki(handle, descs(problems), moduli=12)
+    plan.run()
+    torch.cuda.synchronize()
+    expected = results(problems)
+    for rep in range(3):
+        reset_c(problems)
+        torch.cuda.synchronize()
+        plan.graph_run()
+        torch.cuda.synchronize()
+        assert same(results(problems), expected), f"replay {rep}"
+    plan.close()
+
+
+def test_tuner_keeps_results(handle, problems):
+    import torch
+    import cosma_amd as ca
+    reset_c(problems)
+    plan = ca.GroupedOzaki(handle, descs(problems), moduli=10)
+    plan.run()
+    torch.cuda.synchronize()
+    expected = results(problems)
+    plan.tune()
+    info = plan.info()
+    assert 1 <= info["grid"] <= max(1, info["tiles"])
+    reset_c(problems)
+    plan.graph_run()
+    torch.cuda.synchronize()
+    assert same(results(problems), expected)
+    plan.set_grid(3)
+    reset_c(problems)
+    plan.run()
+    torch.cuda.synchronize()
+    assert same(results(problems), expected)
+    plan.close()
+
+
+def test_non_finite_operands_take_fp64(handle):
+    import torch
+    import cosma_amd as ca
+    probs = make_problems(6, 21, max_dim=90)
+    probs = [p for p in probs if p["k"] > 0 and p["alpha"] != 0.0 and p["A"].any()]
+    bad = probs[0]
+    # an Inf at op(A)(0, 0) (stored entry (0, 0) either way): row 0 of the
+    # product is non-finite, the rest finite
+    bad["A"][0, 0] = np.inf
+    bad["dA"], bad["lda"] = to_dev(bad["A"], bad["lda"])
+    reset_c(probs)
+    plan = ca.GroupedOzaki(handle, descs(probs), moduli=12)
+    plan.run()
+    torch.cuda.synchronize()
+    got = results(probs)
+    assert not np.isfinite(got[0][0]).all()
+    assert np.isfinite(got[0][1:]).all()
+    ref = oracle_result(bad)
+    assert oracle.rel_fro(got[0][1:], ref[1:]) <= 2e-14
+    for p, g in zip(probs[1:], got[1:]):
+        assert oracle.rel_fro(g, oracle_result(p)) <= 1e-11
+    plan.close()
+
+
+def test_invalid_plans_are_rejected(handle):
+    import torch
+    import cosma_amd as ca
+    a = torch.zeros((8, 8), dtype=torch.float64, device="cuda")
+    good = ("N", "N", 8, 8, 8, 1.0, a.data_ptr(), 8, a.data_ptr(), 8, 0.0, a.data_ptr(), 8)
+    with pytest.raises(ca.CosmaAmdError, match="moduli"):
+        ca.GroupedOzaki(handle, [good], moduli=21)
+    host = np.zeros((8, 8))
+    with pytest.raises(ca.CosmaAmdError, match="device memory"):
+        ca.GroupedOzaki(handle, [("N", "N", 8, 8, 8, 1.0, host.ctypes.data, 8, a.data_ptr(), 8, 0.0,
+                                  a.data_ptr(), 8)])
+    empty = ca.GroupedOzaki(handle, [])
+    empty.run()
+    empty.graph_run()
+    empty.close()
