@@ -413,6 +413,133 @@ __global__ __launch_bounds__(NT) void dgemm_mfma_kernel(
     gemm_tile<BM, BN, TA, TB, VEC>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, vec_c, m0, n0, sm);
 }
 
+// Grouped tiles without LDS ("direct"): one wave per 64 x 64 C tile (up to
+// 4 x 4 blocks of 16 x 16 in AGPR accumulators; blocks outside C are skipped),
+// fragments loaded straight from global memory (L1 / L2 cached) one 16-deep
+// k-step ahead of their MFMAs; no LDS and no barriers, so small problems pay
+// neither the staging round trip nor the workgroup synchronisation.  The
+// order of the products in every accumulator is gemm_tile's (per 16-deep
+// step: k = 2g, then 2g + 1, then 8 + 2g, then 9 + 2g in lane group g, zero
+// beyond K), the A rows are permuted the same way and the epilogue is the
+// same, so the results are bit-identical to gemm_tile's.
+template <bool TA, bool TB>
+__device__ __forceinline__ void direct_tile(const GroupedProblem &q, int64_t m0, int64_t n0) {
+    const int lane = threadIdx.x & 63, g = lane >> 4, li = lane & 15;
+    const int arow = 4 * (li & 3) + (li >> 2);
+    const int64_t M = q.m, N = q.n;
+    const bool scale_only = (q.k == 0 || q.alpha == 0.0);
+    const int64_t K = scale_only ? 0 : q.k;
+    const double alpha = scale_only ? 0.0 : q.alpha;
+    const int rows_in = (int)min((int64_t)4, (M - m0 + 15) / 16);
+    const int cols_in = (int)min((int64_t)4, (N - n0 + 15) / 16);
+    const double *__restrict__ A = q.a;
+    const double *__restrict__ B = q.b;
+    f64x4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(0.0, 0.0, f64x4{0.0, 0.0, 0.0, 0.0}, 0, 0, 0);
+    const int koff[4] = {2 * g, 2 * g + 1, 8 + 2 * g, 9 + 2 * g};
+    auto load_step = [&](int64_t k0, double (&a)[4][4], double (&b)[4][4]) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int64_t r = m0 + 16 * i + arow;
+#pragma unroll
+            for (int x = 0; x < 4; ++x) {
+                const int64_t k = k0 + koff[x];
+                a[i][x] = (i < rows_in && r < M && k < K) ? (TA ? A[k + r * q.lda] : A[r + k * q.lda]) : 0.0;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int64_t c = n0 + 16 * j + li;
+#pragma unroll
+            for (int x = 0; x < 4; ++x) {
+                const int64_t k = k0 + koff[x];
+                b[j][x] = (j < cols_in && c < N && k < K) ? (TB ? B[c + k * q.ldb] : B[k + c * q.ldb]) : 0.0;
+            }
+        }
+    };
+    auto mfma_step = [&](const double (&a)[4][4], const double (&b)[4][4]) {
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (i < rows_in && j < cols_in)
+                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][x], b[j][x], acc[i][j], 0, 0, 0);
+    };
+    const int64_t nsteps = (K + 15) / 16;
+    double a0[4][4], b0[4][4], a1[4][4], b1[4][4];
+    if (nsteps > 0) load_step(0, a0, b0);
+    for (int64_t s = 0; s < nsteps; s += 2) {
+        if (s + 1 < nsteps) load_step(16 * (s + 1), a1, b1);
+        mfma_step(a0, b0);
+        if (s + 2 < nsteps) load_step(16 * (s + 2), a0, b0);
+        if (s + 1 < nsteps) mfma_step(a1, b1);
+    }
+    // ---- epilogue (gemm_tile's): lane holds rows 4g..4g+3 of column li
+    const bool beta0 = (q.beta == 0.0);
+    const double beta = q.beta;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int64_t row = m0 + 16 * i + 4 * g;
+        if (row >= M) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int64_t col = n0 + 16 * j + li;
+            if (col >= N) continue;
+            double *cp = q.c + row + col * q.ldc;
+            const f64x4 r = acc[i][j];
+            if (q.vec_c && row + 3 < M) {
+                f64x2 *cv = reinterpret_cast<f64x2 *>(cp);
+                f64x2 lo = {alpha * r[0], alpha * r[1]};
+                f64x2 hi = {alpha * r[2], alpha * r[3]};
+                if (!beta0) {
+                    const f64x2 clo = cv[0], chi = cv[1];
+                    lo.x = fma(beta, clo.x, lo.x); lo.y = fma(beta, clo.y, lo.y);
+                    hi.x = fma(beta, chi.x, hi.x); hi.y = fma(beta, chi.y, hi.y);
+                }
+                cv[0] = lo;
+                cv[1] = hi;
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    if (row + e < M) {
+                        double val = alpha * r[e];
+                        if (!beta0) val = fma(beta, cp[e], val);
+                        cp[e] = val;
+                    }
+                }
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(64) void dgemm_grouped_direct(const GroupedProblem *__restrict__ probs, int nprob,
+                                                           int64_t total_tiles) {
+    for (int64_t t = blockIdx.x; t < total_tiles; t += gridDim.x) {
+        int lo = 0, hi = nprob - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) / 2;
+            if (probs[mid].tile_start <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const GroupedProblem &q = probs[lo];
+        const int64_t local = t - q.tile_start;
+        const int64_t tiles_m = (q.m + 63) / 64;
+        const int64_t m0 = (local % tiles_m) * 64, n0 = (local / tiles_m) * 64;
+        switch (q.flags & 3) {   // bit 0: A transposed, bit 1: B transposed
+        case 0: direct_tile<false, false>(q, m0, n0); break;
+        case 1: direct_tile<true, false>(q, m0, n0); break;
+        case 2: direct_tile<false, true>(q, m0, n0); break;
+        default: direct_tile<true, true>(q, m0, n0); break;
+        }
+    }
+}
+
 // C <- beta * C; beta == 0 writes zeros without reading C.
 // Elementwise kernels over an M x N column-major block: threads run down the
 // rows of a column (coalesced), blocks stride over the columns.
@@ -577,6 +704,7 @@ int grouped_tile_shape(int cfg, int *bm, int *bn) {
     case 1: *bm = 64; *bn = 64; return COSMA_AMD_OK;
     case 2: *bm = 128; *bn = 64; return COSMA_AMD_OK;
     case 3: *bm = 128; *bn = 128; return COSMA_AMD_OK;
+    case 4: *bm = 64; *bn = 64; return COSMA_AMD_OK;   // direct: one wave per tile, no LDS
     default:
         set_error("grouped dgemm: unknown tile configuration %d", cfg);
         return COSMA_AMD_ERR_INVALID_ARG;
@@ -600,6 +728,7 @@ int launch_grouped(int cfg, const GroupedProblem *probs, int nprob, int64_t tota
     case 1: hipLaunchKernelGGL((dgemm_grouped_kernel<64, 64>), g, b, 0, s, probs, nprob, total_tiles); break;
     case 2: hipLaunchKernelGGL((dgemm_grouped_kernel<128, 64>), g, b, 0, s, probs, nprob, total_tiles); break;
     case 3: hipLaunchKernelGGL((dgemm_grouped_kernel<128, 128>), g, b, 0, s, probs, nprob, total_tiles); break;
+    case 4: hipLaunchKernelGGL(dgemm_grouped_direct, g, dim3(64), 0, s, probs, nprob, total_tiles); break;
     default:
         set_error("grouped dgemm: unknown tile configuration %d", cfg);
         return COSMA_AMD_ERR_INVALID_ARG;

@@ -28,8 +28,10 @@ struct cosma_amd_grouped_s {
     cosma_amd_handle_t h = nullptr;
     int nprob = 0;
     uint64_t signature = 0;
-    cosma_amd::GroupedProblem *dev[4] = {nullptr, nullptr, nullptr, nullptr};   // per tile configuration
-    int64_t total_tiles[4] = {0, 0, 0, 0};
+    // per tile configuration 1..4; configuration 4 (64 x 64, direct) shares
+    // configuration 1's descriptors
+    cosma_amd::GroupedProblem *dev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    int64_t total_tiles[5] = {0, 0, 0, 0, 0};
     int cfg = 1;
     int grid = 1;
     hipStream_t capture = nullptr;
@@ -82,8 +84,8 @@ int grouped_create(cosma_amd_handle_t h, const cosma_amd_gemm_desc *d, int64_t c
         set_error("grouped dgemm: invalid problem list");
         return COSMA_AMD_ERR_INVALID_ARG;
     }
-    if (tile_config < 0 || tile_config > 3) {
-        set_error("grouped dgemm: tile_config %d is not 0..3", tile_config);
+    if (tile_config < 0 || tile_config > 4) {
+        set_error("grouped dgemm: tile_config %d is not 0..4", tile_config);
         return COSMA_AMD_ERR_INVALID_ARG;
     }
     std::vector<GroupedProblem> probs;
@@ -163,6 +165,8 @@ int grouped_create(cosma_amd_handle_t h, const cosma_amd_gemm_desc *d, int64_t c
         grouped_destroy(p);
         return rc;
     }
+    p->dev[4] = p->dev[1];
+    p->total_tiles[4] = p->total_tiles[1];
     {
         std::lock_guard<std::mutex> lock(g_tune_mutex);
         auto it = g_tuned.find(p->signature);
@@ -213,7 +217,7 @@ int grouped_tune(cosma_amd_grouped_t p, hipStream_t s) {
     float best = -1.0f;
     int best_cfg = p->cfg, best_grid = p->grid;
     int rc = COSMA_AMD_OK;
-    for (int cfg = 1; cfg <= 3 && rc == COSMA_AMD_OK; ++cfg) {
+    for (int cfg = 1; cfg <= 4 && rc == COSMA_AMD_OK; ++cfg) {
         const int64_t tiles = p->total_tiles[cfg];
         std::vector<int64_t> grids;
         for (int per_cu : {1, 2, 4, 8}) grids.push_back(std::min<int64_t>(tiles, (int64_t)p->h->n_cu * per_cu));
@@ -248,8 +252,8 @@ int grouped_tune(cosma_amd_grouped_t p, hipStream_t s) {
 }
 
 int grouped_set_config(cosma_amd_grouped_t p, int tile_config, int grid) {
-    if (tile_config < 1 || tile_config > 3) {
-        set_error("grouped dgemm: tile_config %d is not 1..3", tile_config);
+    if (tile_config < 1 || tile_config > 4) {
+        set_error("grouped dgemm: tile_config %d is not 1..4", tile_config);
         return COSMA_AMD_ERR_INVALID_ARG;
     }
     const int g = grid > 0 ? (int)std::min<int64_t>(grid, std::max<int64_t>(1, p->total_tiles[tile_config]))
@@ -271,8 +275,8 @@ void grouped_destroy(cosma_amd_grouped_t p) {
     hipSetDevice(p->h->device);
     drop_graph(p);
     if (p->capture) hipStreamDestroy(p->capture);
-    for (auto *d : p->dev)
-        if (d) hipFree(d);
+    for (int cfg = 1; cfg <= 3; ++cfg)   // dev[4] aliases dev[1]
+        if (p->dev[cfg]) hipFree(p->dev[cfg]);
     delete p;
 }
 
