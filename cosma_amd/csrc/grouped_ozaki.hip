@@ -154,14 +154,16 @@ __global__ __launch_bounds__(256) void gz_exponents(const OzProblem *probs, int 
 
 // Plan creation only: the error factor gamma_p of each problem (ozaki.hip's
 // estimate: relative Frobenius error ~ gamma 2^-P), +inf when it cannot be
-// formed, 0 for a zero product.  Reads the exponents written by gz_exponents.
+// formed, 0 for a zero product and for problems that do not use the moduli
+// (scale-only, or flagged non-finite: those are computed in FP64).  Reads the
+// exponents written by gz_exponents.
 __global__ __launch_bounds__(256) void gz_gamma(const OzProblem *probs, const int *flags, double *gamma) {
     const OzProblem &q = probs[blockIdx.x];
     __shared__ int emax[2];
     __shared__ double part[5][256];
     const int t = threadIdx.x;
     if (q.scale_only || flags[blockIdx.x]) {
-        if (t == 0) gamma[blockIdx.x] = q.scale_only ? 0.0 : INFINITY;
+        if (t == 0) gamma[blockIdx.x] = 0.0;
         return;
     }
     if (t < 2) emax[t] = INT_MIN;

@@ -182,6 +182,10 @@ def test_non_finite_operands_take_fp64(handle):
     bad["A"][0, 0] = np.inf
     bad["dA"], bad["lda"] = to_dev(bad["A"], bad["lda"])
     reset_c(probs)
+    # the non-finite problem does not use the moduli: it does not drive the count
+    auto = ca.GroupedOzaki(handle, descs(probs))
+    assert auto.info()["moduli"] < 20 and np.isfinite(auto.info()["error_estimate"])
+    auto.close()
     plan = ca.GroupedOzaki(handle, descs(probs), moduli=12)
     plan.run()
     torch.cuda.synchronize()
