@@ -164,29 +164,57 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void
     const int mod = jobs.modulus[z];
     const float inv_m = 1.0f / (float)(mod > 0 ? mod : 1);
     const double scale = jobs.scale[z];
+    if constexpr (EPI == EPI_RES8) {
+        // The 256 x 256 int8 residues go through LDS (stage 0 of A, 64 KB,
+        // free once every wave has left the main loop) so that the global
+        // writes are whole 256-byte column runs instead of 16-byte pieces in
+        // 16 columns per instruction.  Column c of the tile occupies bytes
+        // [256 c, 256 c + 256) with row r at r ^ (16 (c & 15)): the 4-byte
+        // writes of a wave (16 columns x 4 row quads) hit 64 different banks,
+        // and each 16-byte chunk stays a contiguous chunk.
+        uint8_t *tile = reinterpret_cast<uint8_t *>(&sa[0][0]);
+        __syncthreads();
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int64_t row = m0 + wm * 128 + 16 * i + 4 * g;
-        if (row >= p.M) continue;
+        for (int i = 0; i < 8; ++i) {
+            const int rt = wm * 128 + 16 * i + 4 * g;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int64_t col = n0 + wn * 128 + 16 * j + li;
-            if (col >= p.N) continue;
-            const i32x4 r = acc[i][j];
-            const int64_t off = (int64_t)z * p.out_stride + row + col * p.ldo;
-            const bool full = row + 3 < p.M;
-            if (EPI == EPI_RES8) {
-                int8_t *o = static_cast<int8_t *>(p.out) + off;
-                int8_t v[4];
-                for (int e = 0; e < 4; ++e) v[e] = (int8_t)sym_mod(r[e], mod, inv_m);
-                if (full) {
-                    *reinterpret_cast<uint32_t *>(o) = (uint32_t)(uint8_t)v[0] | ((uint32_t)(uint8_t)v[1] << 8) |
-                                                      ((uint32_t)(uint8_t)v[2] << 16) | ((uint32_t)(uint8_t)v[3] << 24);
-                } else {
-                    for (int e = 0; e < 4 && row + e < p.M; ++e) o[e] = v[e];
-                }
+            for (int j = 0; j < 8; ++j) {
+                const int ct = wn * 128 + 16 * j + li;
+                const i32x4 r = acc[i][j];
+                uint32_t w = 0;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) w |= (uint32_t)(uint8_t)(int8_t)sym_mod(r[e], mod, inv_m) << (8 * e);
+                *reinterpret_cast<uint32_t *>(tile + ct * 256 + (rt ^ (16 * (ct & 15)))) = w;
+            }
+        }
+        __syncthreads();
+        int8_t *out = static_cast<int8_t *>(p.out) + (int64_t)z * p.out_stride;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int id = t + NT * q, ct = id >> 4, c = id & 15;
+            const int64_t col = n0 + ct, row = m0 + 16 * c;
+            if (col >= p.N || row >= p.M) continue;
+            const i32x4 v = *reinterpret_cast<const i32x4 *>(tile + ct * 256 + 16 * (c ^ (ct & 15)));
+            int8_t *o = out + row + col * p.ldo;
+            if (row + 16 <= p.M) {
+                *reinterpret_cast<i32x4 *>(o) = v;
             } else {
-                double *o = static_cast<double *>(p.out) + off;
+                union { i32x4 q; int8_t b[16]; } u;
+                u.q = v;
+                for (int e = 0; e < 16 && row + e < p.M; ++e) o[e] = u.b[e];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int64_t row = m0 + wm * 128 + 16 * i + 4 * g;
+            if (row >= p.M) continue;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int64_t col = n0 + wn * 128 + 16 * j + li;
+                if (col >= p.N) continue;
+                const i32x4 r = acc[i][j];
+                double *o = static_cast<double *>(p.out) + (int64_t)z * p.out_stride + row + col * p.ldo;
                 for (int e = 0; e < 4 && row + e < p.M; ++e) {
                     const double val = scale * (double)r[e];
                     o[e] = p.accumulate ? o[e] + val : val;
@@ -213,9 +241,8 @@ int launch_i8gemm(const I8Problem &p, const I8Jobs &jobs, int Z, I8Epilogue epi,
         set_error("i8gemm: K must be a multiple of 256 and strides/bases 16-byte aligned");
         return COSMA_AMD_ERR_INVALID_ARG;
     }
-    if (epi == EPI_RES8 && (p.ldo % 4 || p.out_stride % 4 ||
-                                                 (reinterpret_cast<uintptr_t>(p.out) & 3))) {
-        set_error("i8gemm: output leading dimension must be a multiple of 4");
+    if (epi == EPI_RES8 && (p.ldo % 16 || p.out_stride % 16 || (reinterpret_cast<uintptr_t>(p.out) & 15))) {
+        set_error("i8gemm: int8 output leading dimension and stride must be multiples of 16 (16-byte aligned base)");
         return COSMA_AMD_ERR_INVALID_ARG;
     }
     for (int z = 0; z < Z; ++z) {

@@ -58,8 +58,8 @@ struct I8Problem {
 };
 
 // Requirements: K % 256 == 0; lda, ldb, a_stride, b_stride multiples of 16 and
-// 16-byte aligned bases; int8 outputs with ldo and out_stride multiples of 4
-// (all checked).  Not checkable, hence the callers' duty: every A slice
+// 16-byte aligned bases; int8 outputs with ldo and out_stride multiples of 16
+// and a 16-byte aligned base (all checked).  Not checkable, hence the callers' duty: every A slice
 // holds ceil(M/256)*256 rows and every B slice ceil(N/256)*256 columns (the
 // split kernels pad with zeros), so tile loads never need bounds checks.
 int launch_i8gemm(const I8Problem &p, const I8Jobs &jobs, int Z, I8Epilogue epi, hipStream_t s);

@@ -427,7 +427,7 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
         return COSMA_AMD_ERR_UNSUPPORTED;
     }
     const int64_t Mp = round_up(m, 256), Np = round_up(n, 256), Kp = round_up(k, 256);
-    const int64_t ldr = round_up(m, 8);                         // residue output ld
+    const int64_t ldr = round_up(m, 16);                        // residue output ld
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     auto grow = [](DevBuf &buf, size_t need) -> int {
         if (buf.bytes >= need) return COSMA_AMD_OK;
