@@ -23,7 +23,7 @@
 // residues of op(A) (N planes of Mp x Kp bytes, row r of op(A) k-contiguous),
 // of op(B) (N planes of Np x Kp, column j k-contiguous), and of the products
 // (N planes of ldr x n), where Mp, Np, Kp are m, n, k rounded up to 64 and
-// ldr = m rounded up to 4; then the exponents E (Mp) and F (Np) and one
+// ldr = m rounded up to 16; then the exponents E (Mp) and F (Np) and one
 // non-finite flag per problem.  A problem whose operands hold Inf or NaN is
 // computed by plain FP64 dot products in the recombination kernel, so IEEE
 // propagation is kept.
@@ -297,20 +297,40 @@ __global__ __launch_bounds__(256) void gz_products(const OzProblem *probs, int n
                 for (int j = 0; j < 2; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[i], fb[j], acc[i][j], 0, 0, 0);
         }
-        // lane 16 g + li holds rows 4g .. 4g + 3 of column li of each 16 x 16 block
+        // lane 16 g + li holds rows 4g .. 4g + 3 of column li of each 16 x 16
+        // block.  The 64 x 64 residues go through LDS (column c of the tile at
+        // bytes [80 c, 80 c + 64): the 4-byte writes of a wave hit 64
+        // different banks) and leave as 16-byte pieces, four per column.
         const int mod = md.m[l];
         const float inv = md.inv_mf[l];
-        int8_t *O = q.ores + (int64_t)l * q.oplane;
+        uint8_t *tile = reinterpret_cast<uint8_t *>(&sa[0][0]);
+        __syncthreads();   // every wave's last fragment reads are done
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
-                const int64_t row = m0 + wm * 32 + 16 * i + 4 * g, col = n0 + wn * 32 + 16 * j + li;
-                if (col >= q.n) continue;
+                const int rt = wm * 32 + 16 * i + 4 * g, ct = wn * 32 + 16 * j + li;
+                uint32_t w = 0;
 #pragma unroll
-                for (int e = 0; e < 4; ++e)
-                    if (row + e < q.m) O[row + e + col * q.ldr] = (int8_t)sym_mod(acc[i][j][e], mod, inv);
+                for (int e = 0; e < 4; ++e) w |= (uint32_t)(uint8_t)(int8_t)sym_mod(acc[i][j][e], mod, inv) << (8 * e);
+                *reinterpret_cast<uint32_t *>(tile + ct * GZ_ROW + rt) = w;
             }
+        __syncthreads();
+        {
+            const int ct = t >> 2, c = t & 3;
+            const int64_t col = n0 + ct, row = m0 + 16 * c;
+            if (col < q.n && row < q.m) {
+                const i32x4 v = *reinterpret_cast<const i32x4 *>(tile + ct * GZ_ROW + 16 * c);
+                int8_t *o = q.ores + (int64_t)l * q.oplane + row + col * q.ldr;
+                if (row + 16 <= q.m) {
+                    *reinterpret_cast<i32x4 *>(o) = v;
+                } else {
+                    union { i32x4 q; int8_t b[16]; } u;
+                    u.q = v;
+                    for (int e = 0; e < 16 && row + e < q.m; ++e) o[e] = u.b[e];
+                }
+            }
+        }
     }
 }
 
@@ -527,7 +547,7 @@ int grouped_ozaki_create(cosma_amd_handle_t h, const cosma_amd_gemm_desc *d, int
         g.Mp = up(q.m, GZ_TILE);
         g.Np = up(q.n, GZ_TILE);
         g.Kp = scale_only ? 0 : up(q.k, 64);
-        g.ldr = up(q.m, 4);
+        g.ldr = up(q.m, 16);
         probs.push_back(g);
         cost.push_back(scale_only ? 0.0 : (double)q.m * q.n * q.k);
     }
