@@ -233,11 +233,16 @@ __global__ __launch_bounds__(256) void gz_split(const OzProblem *probs, GzModuli
     for (int64_t c = threadIdx.x; c < chunks; c += blockDim.x) {
         const int64_t r = contig ? c / nkc : c % Rp;
         const int64_t k0 = 16 * (contig ? c % nkc : c / Rp);
-        double v[16];
         const bool rin = r < R;
+        if (!rin || k0 >= q.k) {   // padding: residues of zero
+            for (int l = 0; l < md.count; ++l)
+                *reinterpret_cast<i32x4 *>(dst + r * q.Kp + k0 + (int64_t)l * plane) = i32x4{0, 0, 0, 0};
+            continue;
+        }
+        double v[16];
 #pragma unroll
-        for (int j = 0; j < 16; ++j) v[j] = (rin && k0 + j < q.k) ? vec_at(x, ld, contig, r, k0 + j) : 0.0;
-        moduli_residues16(v, rin ? ex[r] : 0, q.P, md.count, md.m, md.inv_m, dst + r * q.Kp + k0, plane);
+        for (int j = 0; j < 16; ++j) v[j] = k0 + j < q.k ? vec_at(x, ld, contig, r, k0 + j) : 0.0;
+        moduli_residues16(v, ex[r], q.P, md.count, md.m, md.inv_m, dst + r * q.Kp + k0, plane);
     }
 }
 
