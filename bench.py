@@ -210,7 +210,8 @@ def dd_rows(Ad, Bd, n, rows):
 
 def split_precision(phis=(0.0, 1.0, 4.0), steps=5, warmup=1, n=8192, rows=(0, 4097)):
     """8192^3 C = A*B (NN): plain FP64 MFMA DGEMM vs the split-precision DGEMM
-    (moduli at the default 1e-14 tolerance; slices with 7 slices), inputs
+    (moduli at the default 1e-14 tolerance, with INT8 and with BF16 residues;
+    slices with 7 slices), inputs
     a = (u - 0.5) exp(phi z).  Errors are relative Frobenius errors of sampled
     rows against the double-double oracle."""
     import torch
@@ -242,7 +243,8 @@ def split_precision(phis=(0.0, 1.0, 4.0), steps=5, warmup=1, n=8192, rows=(0, 40
         e_plain = rows_error(A, B, C, n, rows, ref)
         entry = {"phi": phi, "fp64": {"tflops": round(flops / t_plain / 1e12, 3), "ms": round(t_plain * 1e3, 3),
                                       "rel_err": e_plain}}
-        for name, method, count in (("moduli", ca.OZAKI_MODULI, 0), ("slices", ca.OZAKI_SLICES, 7)):
+        for name, method, count in (("moduli", ca.OZAKI_MODULI, 0), ("moduli_bf16", ca.OZAKI_MODULI_BF16, 0),
+                                    ("slices", ca.OZAKI_SLICES, 7)):
             info = {}
 
             def oz():

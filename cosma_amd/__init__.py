@@ -37,6 +37,7 @@ class CosmaAmdError(RuntimeError):
 
 OZAKI_MODULI = 0
 OZAKI_SLICES = 1
+OZAKI_MODULI_BF16 = 2
 
 
 class Block(ctypes.Structure):

@@ -158,7 +158,8 @@ int cosma_amd_dgemm_ozaki(cosma_amd_handle_t h, char transa, char transb,
     const char ta = norm_trans(transa), tb = norm_trans(transb);
     CA_CHECK(ta != 0, "dgemm_ozaki: invalid transa '%c'", transa);
     CA_CHECK(tb != 0, "dgemm_ozaki: invalid transb '%c'", transb);
-    CA_CHECK(method == COSMA_AMD_OZAKI_MODULI || method == COSMA_AMD_OZAKI_SLICES,
+    CA_CHECK(method == COSMA_AMD_OZAKI_MODULI || method == COSMA_AMD_OZAKI_SLICES ||
+                 method == COSMA_AMD_OZAKI_MODULI_BF16,
              "dgemm_ozaki: unknown method %d", method);
     CA_CHECK(m >= 0 && n >= 0 && k >= 0, "dgemm_ozaki: negative extent");
     if (m == 0 || n == 0) return COSMA_AMD_OK;

@@ -1,5 +1,7 @@
 // Batched INT8 MFMA GEMM with pair lists and residue / FP64 epilogues.
 // See i8gemm.hpp for the contract and the tile layout.
+#include <type_traits>
+
 #include "i8gemm.hpp"
 
 namespace cosma_amd {
@@ -15,8 +17,22 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t T) {
     return (xcd < r) ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
 }
 
-template <int EPI>
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// One MFMA on 16-byte fragments: INT8 (64 k, INT32 sums) or BF16 (32 k,
+// FP32 sums).
+template <bool BF16>
+__device__ __forceinline__ auto mma(const i32x4 &a, const i32x4 &b, const std::conditional_t<BF16, f32x4, i32x4> &c) {
+    if constexpr (BF16)
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                       0, 0, 0);
+    else
+        return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
+}
+
+template <int EPI, bool BF16>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void i8gemm_kernel(I8Problem p, I8Jobs jobs) {
+    using acc_t = std::conditional_t<BF16, f32x4, i32x4>;
     // LDS: two stages of 256 x 128-byte rows per operand (128 KB in all).
     // Row r holds its eight 16-byte chunks in the order chunk ^ (r & 7): the
     // stores (eight consecutive lanes per row, one full 128-byte cache line
@@ -43,12 +59,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void
     const int g = lane >> 4, li = lane & 15;
 
     // accumulators zeroed by an MFMA so they live in AGPRs across the loop
-    i32x4 acc[8][8];
+    acc_t acc[8][8];
     const i32x4 zero4 = {0, 0, 0, 0};
+    const acc_t zacc = {0, 0, 0, 0};
 #pragma unroll
     for (int i = 0; i < 8; ++i)
 #pragma unroll
-        for (int j = 0; j < 8; ++j) acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(zero4, zero4, zero4, 0, 0, 0);
+        for (int j = 0; j < 8; ++j) acc[i][j] = mma<BF16>(zero4, zero4, zacc);
 
     const int ksteps = (int)(p.K / BK);
     const int nsteps = jobs.npairs[z] * ksteps;
@@ -104,7 +121,39 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void
         for (int i = 0; i < 8; ++i)
 #pragma unroll
             for (int j = 0; j < 8; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[h][i], fb[h][j], acc[i][j], 0, 0, 0);
+                acc[i][j] = mma<BF16>(fa[h][i], fb[h][j], acc[i][j]);
+    };
+    // BF16: the exact FP32 sums are reduced modulo the job's modulus (see
+    // i8gemm.hpp), x <- x - m * rint(x / m), exact on integers below 2^24.
+    // Written as plain FP32 code the reduction made the compiler move the
+    // accumulators out of the AGPRs (500+ spilled registers), so each element
+    // is read, reduced and written back in place by a short asm block.  The
+    // hazard recognizer does not look into asm, so the required wait states
+    // are inserted by hand: s_nops before the first read (MFMA result ->
+    // VALU read) and after the last write (VALU AGPR write -> MFMA SrcC read).
+    const float fmod_m = (float)(jobs.modulus[z] > 0 ? jobs.modulus[z] : 1);
+    const float neg_m = -fmod_m, finv_m = 1.0f / fmod_m;
+    auto reduce = [&]() {
+        if constexpr (BF16) {
+            asm volatile("s_nop 7\n\ts_nop 7\n\ts_nop 7" ::: "memory");
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        float x = acc[i][j][e], t, q;
+                        asm volatile("v_accvgpr_read_b32 %1, %0\n\t"
+                                     "v_mul_f32 %2, %1, %4\n\t"
+                                     "v_rndne_f32 %2, %2\n\t"
+                                     "v_fmac_f32 %1, %2, %3\n\t"
+                                     "v_accvgpr_write_b32 %0, %1"
+                                     : "+a"(x), "=&v"(t), "=&v"(q)
+                                     : "v"(neg_m), "v"(finv_m));
+                        acc[i][j][e] = x;
+                    }
+            asm volatile("s_nop 7\n\ts_nop 7" ::: "memory");
+        }
     };
     const int last = nsteps - 1;
     // Step s lives in LDS stage s & 1.  Entering iteration s, the registers
@@ -157,6 +206,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void
         for (int s = 0; s < nsteps; s += 2) {
             iteration(s, 0);
             iteration(s + 1, 1);
+            if (BF16 && (s + 2) % BF16_FLUSH_STEPS == 0) reduce();
         }
     }
 
@@ -180,7 +230,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 const int ct = wn * 128 + 16 * j + li;
-                const i32x4 r = acc[i][j];
+                i32x4 r;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) r[e] = (int)acc[i][j][e];   // exact integers below 2^24 (BF16)
                 uint32_t w = 0;
 #pragma unroll
                 for (int e = 0; e < 4; ++e) w |= (uint32_t)(uint8_t)(int8_t)sym_mod(r[e], mod, inv_m) << (8 * e);
@@ -213,7 +265,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void
             for (int j = 0; j < 8; ++j) {
                 const int64_t col = n0 + wn * 128 + 16 * j + li;
                 if (col >= p.N) continue;
-                const i32x4 r = acc[i][j];
+                const acc_t r = acc[i][j];
                 double *o = static_cast<double *>(p.out) + (int64_t)z * p.out_stride + row + col * p.ldo;
                 for (int e = 0; e < 4 && row + e < p.M; ++e) {
                     const double val = scale * (double)r[e];
@@ -241,6 +293,10 @@ int launch_i8gemm(const I8Problem &p, const I8Jobs &jobs, int Z, I8Epilogue epi,
         set_error("i8gemm: K must be a multiple of 256 and strides/bases 16-byte aligned");
         return COSMA_AMD_ERR_INVALID_ARG;
     }
+    if (p.bf16 && epi != EPI_RES8) {
+        set_error("i8gemm: BF16 operands need the residue epilogue");
+        return COSMA_AMD_ERR_INVALID_ARG;
+    }
     if (epi == EPI_RES8 && (p.ldo % 16 || p.out_stride % 16 || (reinterpret_cast<uintptr_t>(p.out) & 15))) {
         set_error("i8gemm: int8 output leading dimension and stride must be multiples of 16 (16-byte aligned base)");
         return COSMA_AMD_ERR_INVALID_ARG;
@@ -258,8 +314,11 @@ int launch_i8gemm(const I8Problem &p, const I8Jobs &jobs, int Z, I8Epilogue epi,
     const int64_t tiles = ((p.M + BM - 1) / BM) * ((p.N + BN - 1) / BN);
     const dim3 grid((unsigned)tiles, (unsigned)Z), block(NT);
     switch (epi) {
-    case EPI_RES8: hipLaunchKernelGGL(i8gemm_kernel<EPI_RES8>, grid, block, 0, s, p, jobs); break;
-    default: hipLaunchKernelGGL(i8gemm_kernel<EPI_F64ACC>, grid, block, 0, s, p, jobs); break;
+    case EPI_RES8:
+        if (p.bf16) hipLaunchKernelGGL((i8gemm_kernel<EPI_RES8, true>), grid, block, 0, s, p, jobs);
+        else hipLaunchKernelGGL((i8gemm_kernel<EPI_RES8, false>), grid, block, 0, s, p, jobs);
+        break;
+    default: hipLaunchKernelGGL((i8gemm_kernel<EPI_F64ACC, false>), grid, block, 0, s, p, jobs); break;
     }
     CA_HIP(hipGetLastError());
     return COSMA_AMD_OK;

@@ -25,6 +25,15 @@
 // MFMA i32 16x16x64 lane layout (checked on gfx950): lane l = 16g + i supplies
 // A row i / B column i, bytes k = 16g .. 16g+15; it receives D rows 4g..4g+3 of
 // column i.
+//
+// BF16 operands (I8Problem::bf16, EPI_RES8 only): the same byte layout holds
+// BF16 numbers with integer values in [-128, 127] (K, lda, ldb and the slice
+// strides still count bytes, two per element), and the same 16-byte
+// fragments feed v_mfma_f32_16x16x32_bf16.  Products of such integers are
+// below 2^14 and FP32 holds every integer below 2^24 exactly, so the FP32
+// accumulators stay exact as long as they are reduced modulo the job's
+// modulus before they can reach 2^24: every BF16_FLUSH_STEPS k-steps (896
+// elements: 896 * 2^14 plus a reduced value below 384 is under 2^24).
 #pragma once
 #include "common.hpp"
 
@@ -55,9 +64,12 @@ struct I8Problem {
     int64_t ldo;          // leading dimension of the outputs (elements)
     int64_t out_stride;   // elements between the outputs of consecutive jobs
     int accumulate;       // EPI_F64ACC: add into the output
+    int bf16;             // operands are BF16 integers (EPI_RES8, one pair per job)
 };
 
-// Requirements: K % 256 == 0; lda, ldb, a_stride, b_stride multiples of 16 and
+constexpr int BF16_FLUSH_STEPS = 14;
+
+// Requirements: K % 256 == 0 (bytes); lda, ldb, a_stride, b_stride multiples of 16 and
 // 16-byte aligned bases; int8 outputs with ldo and out_stride multiples of 16
 // and a 16-byte aligned base (all checked).  Not checkable, hence the callers' duty: every A slice
 // holds ceil(M/256)*256 rows and every B slice ceil(N/256)*256 columns (the

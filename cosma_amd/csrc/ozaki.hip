@@ -148,10 +148,12 @@ struct SplitArgs {
     const double *x;
     int64_t ld;
     int64_t R, K;          // extents of op(X) (rows, shared dim)
-    int64_t Kpad;          // row stride of the int8 output (bytes)
+    int64_t Kpad;          // padded k (elements)
+    int64_t row_bytes;     // row stride of the output (bytes)
     int64_t slice_stride;  // bytes between slices / residues
     const int *E;
-    int method;            // 0 moduli, 1 slices
+    int method;            // 0 moduli, 1 slices, 2 moduli with BF16 residues
+    int esize;             // bytes per output element (2 for BF16)
     int count;             // number of moduli / slices
     int P;                 // bits of the integer inputs (moduli)
     int m[MAX_MODULI];
@@ -161,8 +163,10 @@ struct SplitArgs {
 
 // 16 consecutive k of one row -> 16 bytes per modulus / slice
 __device__ __forceinline__ void split16(const SplitArgs &a, const double (&v)[16], int e, int8_t *dst) {
-    if (a.method == 0) {
-        moduli_residues16(v, e, a.P, a.count, a.m, a.inv_m, dst, a.slice_stride);
+    if (a.method == COSMA_AMD_OZAKI_MODULI) {
+        moduli_residues16<false>(v, e, a.P, a.count, a.m, a.inv_m, dst, a.slice_stride);
+    } else if (a.method == COSMA_AMD_OZAKI_MODULI_BF16) {
+        moduli_residues16<true>(v, e, a.P, a.count, a.m, a.inv_m, dst, a.slice_stride);
     } else {
         double r[16];
 #pragma unroll
@@ -203,7 +207,7 @@ __global__ void split_contig(SplitArgs a, int64_t Rpad) {
 #pragma unroll
             for (int j = 0; j < 16; ++j) v[j] = (rin && k0 + j < a.K) ? a.x[k0 + j + r * a.ld] : 0.0;
         }
-        split16(a, v, rin ? a.E[r] : 0, a.out + r * a.Kpad + k0);
+        split16(a, v, rin ? a.E[r] : 0, a.out + r * a.row_bytes + k0 * a.esize);
     }
 }
 
@@ -225,7 +229,7 @@ __global__ __launch_bounds__(256) void split_strided(SplitArgs a) {
 #pragma unroll
     for (int j = 0; j < 16; ++j) v[j] = tile[kc * 16 + j][row];
     const int64_t r = r0 + row;
-    split16(a, v, r < a.R ? a.E[r] : 0, a.out + r * a.Kpad + k0 + kc * 16);
+    split16(a, v, r < a.R ? a.E[r] : 0, a.out + r * a.row_bytes + (k0 + kc * 16) * a.esize);
 }
 
 // ------------------------------------------------------ recombination
@@ -414,6 +418,8 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
     if (k == 0 || alpha == 0.0) return launch_scale(m, n, beta, c, ldc, s);
     if (tol <= 0.0) tol = 1e-14;
     const int slices = method == COSMA_AMD_OZAKI_SLICES;
+    const bool bf16 = method == COSMA_AMD_OZAKI_MODULI_BF16;   // residues as BF16, products on BF16 MFMA
+    const int64_t esize = bf16 ? 2 : 1;
     if (!slices && count > 0 && (count < 2 || count > MAX_MODULI)) {
         set_error("ozaki: %d moduli requested (2..%d supported)", count, MAX_MODULI);
         return COSMA_AMD_ERR_INVALID_ARG;
@@ -422,7 +428,7 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
         set_error("ozaki: %d slices requested (1..%d supported)", count, I8_MAX_PAIRS);
         return COSMA_AMD_ERR_INVALID_ARG;
     }
-    if (!slices && (double)k * 16384.0 >= 2147483648.0) {
+    if (!slices && !bf16 && (double)k * 16384.0 >= 2147483648.0) {
         set_error("ozaki moduli: k = %lld too large for exact INT32 accumulation", (long long)k);
         return COSMA_AMD_ERR_UNSUPPORTED;
     }
@@ -530,7 +536,8 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
     // at addresses 2^26 apart, and the split stores and recombination loads,
     // which touch the same offset of every plane together, would pile onto
     // the same memory channels.
-    const int64_t a_plane = Mp * Kp + PLANE_PAD, b_plane = Np * Kp + PLANE_PAD, r_plane = ldr * n + PLANE_PAD;
+    const int64_t a_plane = Mp * Kp * esize + PLANE_PAD, b_plane = Np * Kp * esize + PLANE_PAD;
+    const int64_t r_plane = ldr * n + PLANE_PAD;
     const size_t a_bytes = (size_t)count * a_plane, b_bytes = (size_t)count * b_plane;
     const size_t out_bytes = slices ? (size_t)ldr * n * 8 : (size_t)count * r_plane;
     if ((rc = grow(h->ozaki_ws, al(a_bytes) + al(b_bytes) + al(out_bytes)))) return rc;
@@ -543,8 +550,9 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
     auto split = [&](const double *x, int64_t ld, int64_t R, int64_t Rpad, bool contig, const int *ex,
                      int8_t *dst, int64_t stride) {
         SplitArgs sa{};
-        sa.x = x; sa.ld = ld; sa.R = R; sa.K = k; sa.Kpad = Kp; sa.slice_stride = stride;
-        sa.E = ex; sa.method = slices; sa.count = count; sa.P = P; sa.out = dst;
+        sa.x = x; sa.ld = ld; sa.R = R; sa.K = k; sa.Kpad = Kp; sa.row_bytes = Kp * esize; sa.slice_stride = stride;
+        sa.E = ex; sa.method = slices ? COSMA_AMD_OZAKI_SLICES : method; sa.esize = (int)esize;
+        sa.count = count; sa.P = P; sa.out = dst;
         for (int l = 0; l < MAX_MODULI; ++l) { sa.m[l] = kModuli[l]; sa.inv_m[l] = 1.0 / kModuli[l]; }
         if (contig)
             hipLaunchKernelGGL(split_contig, dim3(grid_for(Rpad * (Kp / 16))), dim3(256), 0, s, sa, Rpad);
@@ -557,9 +565,10 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
 
     // ---- 3. INT8 products
     I8Problem pr{};
-    pr.M = m; pr.N = n; pr.K = Kp;
-    pr.a = As; pr.lda = Kp; pr.a_stride = a_plane;
-    pr.b = Bs; pr.ldb = Kp; pr.b_stride = b_plane;
+    pr.M = m; pr.N = n; pr.K = Kp * esize;
+    pr.a = As; pr.lda = Kp * esize; pr.a_stride = a_plane;
+    pr.b = Bs; pr.ldb = Kp * esize; pr.b_stride = b_plane;
+    pr.bf16 = bf16;
     pr.out = out; pr.ldo = ldr;
     I8Jobs jobs{};
     int64_t products = 0;
@@ -607,7 +616,7 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
     }
     CA_HIP(hipGetLastError());
     if (info) {
-        info->method = slices ? COSMA_AMD_OZAKI_SLICES : COSMA_AMD_OZAKI_MODULI;
+        info->method = slices ? COSMA_AMD_OZAKI_SLICES : method;
         info->count = count;
         info->bits = slices ? 7 * count : P;
         info->int8_products = products;

@@ -50,9 +50,29 @@ __device__ __forceinline__ int row_exponent(unsigned long long bits, int slices)
     return e;
 }
 
+// 16 residues to a plane: 16 int8 bytes, or (BF16) 16 BF16 numbers of the
+// same integer values in 32 bytes (integers of at most 8 significant bits are
+// exact in BF16: the top half of their FP32 pattern).
+template <bool BF16>
+__device__ __forceinline__ void store_residues16(const i32x4 &q, int8_t *dst) {
+    if constexpr (BF16) {
+        union { i32x4 q; int8_t b[16]; } u;
+        u.q = q;
+        uint32_t w[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            w[j] = (__float_as_uint((float)u.b[2 * j]) >> 16) | (__float_as_uint((float)u.b[2 * j + 1]) & 0xffff0000u);
+        *reinterpret_cast<i32x4 *>(dst) = i32x4{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
+        *reinterpret_cast<i32x4 *>(dst + 16) = i32x4{(int)w[4], (int)w[5], (int)w[6], (int)w[7]};
+    } else {
+        *reinterpret_cast<i32x4 *>(dst) = q;
+    }
+}
+
 // Symmetric residues of 16 consecutive entries v (scaled by 2^(P - e) and
-// rounded to integers) modulo the first `count` moduli: 16 bytes per plane at
-// dst + l * plane.
+// rounded to integers) modulo the first `count` moduli: 16 bytes (BF16: 32)
+// per plane at dst + l * plane.
+template <bool BF16 = false>
 __device__ __forceinline__ void moduli_residues16(const double (&v)[16], int e, int P, int count, const int *mods,
                                                   const double *inv_mods, int8_t *dst, int64_t plane) {
     double iv[16];
@@ -69,7 +89,7 @@ __device__ __forceinline__ void moduli_residues16(const double (&v)[16], int e, 
             union { i32x4 q; int8_t b[16]; } u;
 #pragma unroll
             for (int j = 0; j < 16; ++j) u.b[j] = (int8_t)(int)fma(-m, rint(iv[j] * im), iv[j]);
-            *reinterpret_cast<i32x4 *>(dst + (int64_t)l * plane) = u.q;
+            store_residues16<BF16>(u.q, dst + (int64_t)l * plane);
         }
     } else {
         for (int l = 0; l < count; ++l) {
@@ -83,7 +103,7 @@ __device__ __forceinline__ void moduli_residues16(const double (&v)[16], int e, 
                 if (r < lo) r += mods[l];
                 u.b[j] = (int8_t)r;
             }
-            *reinterpret_cast<i32x4 *>(dst + (int64_t)l * plane) = u.q;
+            store_residues16<BF16>(u.q, dst + (int64_t)l * plane);
         }
     }
 }

@@ -80,14 +80,17 @@ int cosma_amd_dgemm(cosma_amd_handle_t handle, char transa, char transb,
 /* ------------------------------------------------ split-precision GEMM */
 enum {
     COSMA_AMD_OZAKI_MODULI = 0,   /* integer residues modulo coprime moduli <= 256, CRT recombination */
-    COSMA_AMD_OZAKI_SLICES = 1    /* signed 7-bit slices, triangular slice products */
+    COSMA_AMD_OZAKI_SLICES = 1,   /* signed 7-bit slices, triangular slice products */
+    COSMA_AMD_OZAKI_MODULI_BF16 = 2 /* the moduli method with BF16 residues: products on the BF16
+                                       matrix cores (v_mfma_f32_16x16x32_bf16), exact FP32 sums
+                                       reduced modulo each modulus every 896 k */
 };
 
 typedef struct {
     int method;             /* method used; -1 if A or B held Inf/NaN (FP64 MFMA DGEMM used instead) */
     int count;              /* moduli or slices used */
     int bits;               /* bits kept per input entry relative to its row/column maximum */
-    int64_t int8_products;  /* INT8 GEMMs of size m x n x k computed */
+    int64_t int8_products;  /* INT8 (BF16 for COSMA_AMD_OZAKI_MODULI_BF16) GEMMs of size m x n x k computed */
     double error_bound;     /* estimated relative Frobenius error (doubled), from the inputs' row statistics */
 } cosma_amd_ozaki_info;
 
@@ -107,8 +110,13 @@ typedef struct {
  * estimate meets the tolerance is used.  count > 0 forces the number of
  * moduli (2..20) or slices (1..8) instead.  info (may be NULL) receives the
  * method, count, bits and the doubled error estimate.
- * Limits: moduli need k < 131072 and slices need slices * k * 127^2 < 2^31
- * (exact INT32 sums); COSMA_AMD_ERR_UNSUPPORTED otherwise.
+ * The BF16 variant (COSMA_AMD_OZAKI_MODULI_BF16) holds the same residues as
+ * BF16 numbers (exact: integers in [-128, 127]) and multiplies them on the
+ * BF16 matrix cores; it chooses the same count and gives bit-identical
+ * results at twice the operand bytes.
+ * Limits: moduli need k < 131072 (INT8; the BF16 variant has no such limit)
+ * and slices need slices * k * 127^2 < 2^31 (exact INT32 sums);
+ * COSMA_AMD_ERR_UNSUPPORTED otherwise.
  * Uses workspaces owned by the handle; asynchronous except for one read-back
  * of the row statistics (non-finite flag, exponents, sums of squares).
  * No counterpart in the reference (its local multiply is a host or vendor

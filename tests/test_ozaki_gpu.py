@@ -91,7 +91,7 @@ def emulate_slices(Aop, Bop, s, alpha):
     return alpha * np.ldexp(acc, E[:, None] + F[None, :])
 
 
-@pytest.mark.parametrize("method,count", [(0, 0), (1, 3)])
+@pytest.mark.parametrize("method,count", [(0, 0), (1, 3), (2, 0)])
 @pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T")])
 def test_integer_inputs_are_exact(handle, method, count, ta, tb):
     m, n, k = 130, 70, 300
@@ -133,7 +133,7 @@ def test_slices_match_emulation_bit_for_bit(handle, s):
     assert np.array_equal(got, emulate_slices(X, Y, s, 0.75))
 
 
-@pytest.mark.parametrize("method", [0, 1])
+@pytest.mark.parametrize("method", [0, 1, 2])
 @pytest.mark.parametrize("m,n,k,ta,tb", [(256, 256, 1024, "N", "N"), (300, 200, 777, "T", "N"),
                                           (129, 257, 2048, "N", "T"), (512, 64, 4096, "T", "T")])
 def test_default_tolerance_against_double_double(handle, method, m, n, k, ta, tb):
@@ -146,6 +146,61 @@ def test_default_tolerance_against_double_double(handle, method, m, n, k, ta, tb
     err = oracle.rel_fro(got, ref)
     assert err <= 1e-14, (err, info)
     assert err <= info["error_bound"], (err, info)
+
+
+@pytest.mark.parametrize("count", [2, 9, 14, 20])
+@pytest.mark.parametrize("m,n,k,ta,tb", [(37, 45, 129, "N", "N"), (300, 260, 1800, "T", "N"),
+                                          (130, 70, 4000, "N", "T")])
+def test_bf16_residues_bit_identical_to_int8(handle, count, m, n, k, ta, tb):
+    """The BF16 variant holds the same residues and its FP32 sums are exact
+    (reduced modulo m every 896 k), so it must equal the INT8 path bit for
+    bit; k = 1800 and 4000 cross two and four reduction points."""
+    import cosma_amd as ca
+    A = oracle.uniform(*stored_shape(ta, m, k), 11 + count)
+    B = oracle.uniform(*stored_shape(tb, k, n), 12 + count)
+    A[(0, 3), (1, 2)] *= 2.0 ** 30       # entries far above the rest of their row / column
+    C0 = oracle.uniform(m, n, 13)
+    got8, info8 = run_ozaki(handle, ta, tb, m, n, k, -0.7, A, B, 0.5, C0, ca.OZAKI_MODULI, count=count)
+    got16, info16 = run_ozaki(handle, ta, tb, m, n, k, -0.7, A, B, 0.5, C0, ca.OZAKI_MODULI_BF16, count=count)
+    assert info16["method"] == ca.OZAKI_MODULI_BF16 and info16["count"] == count
+    assert info16["bits"] == info8["bits"] and info16["int8_products"] == count
+    assert np.array_equal(got16, got8)
+
+
+def test_bf16_long_sums_of_large_odd_products_stay_exact(handle):
+    """Constant operands: every residue plane is constant, so each FP32 sum
+    adds k copies of the same product r_A * r_B.  For the chosen values some
+    of these products are odd and k * |r_A r_B| exceeds 2^24, where FP32
+    addition without the periodic reduction would drop bits (checked below
+    on the CPU); the result must still be the exact product of the rounded
+    inputs."""
+    import cosma_amd as ca
+    m, n, k, count = 64, 48, 4096, 14
+    x, y = 0.8137, 0.6071
+    A = np.full((m, k), x, order="F")
+    B = np.full((k, n), y, order="F")
+    got, info = run_ozaki(handle, "N", "N", m, n, k, 1.0, A, B, 0.0, np.zeros((m, n), order="F"),
+                          ca.OZAKI_MODULI_BF16, count=count)
+    P = info["bits"]
+    ai, bi = int(np.rint(np.ldexp(x, P))), int(np.rint(np.ldexp(y, P)))
+
+    def sym(v, md):
+        r = v % md
+        return r - md if r > (md - 1) // 2 else r
+    risky = [md for md in MODULI[:count]
+             if (sym(ai, md) * sym(bi, md)) % 2 and k * abs(sym(ai, md) * sym(bi, md)) > 2 ** 24]
+    assert risky, "the operands do not exercise sums beyond 2^24"
+    md = risky[0]
+    prod = np.float32(sym(ai, md) * sym(bi, md))
+    naive = np.float32(0)
+    for _ in range(k):
+        naive = np.float32(naive + prod)
+    assert int(naive) != k * int(prod)          # FP32 alone would be inexact here
+    exact = math.ldexp(float(k * ai * bi), -2 * P)
+    assert np.all(np.abs(got - exact) <= 2 * np.spacing(exact)), (got[0, 0], exact)
+    got8, _ = run_ozaki(handle, "N", "N", m, n, k, 1.0, A, B, 0.0, np.zeros((m, n), order="F"),
+                        ca.OZAKI_MODULI, count=count)
+    assert np.array_equal(got, got8)
 
 
 def test_count_from_tolerance_is_monotone(handle):
