@@ -255,6 +255,26 @@ def split_precision(phis=(0.0, 1.0, 4.0), steps=5, warmup=1, n=8192, rows=(0, 40
                            "speedup_vs_fp64": round(t_plain / t, 3), "rel_err": rows_error(A, B, C, n, rows, ref),
                            "count": info["count"], "bits": info["bits"], "int8_products": info["int8_products"],
                            "error_estimate": info["error_bound"]}
+        # error against the double-double oracle for each count: moduli 8..20
+        # and slices 4..8 (one warm-up and one timed call each)
+        sweep = []
+        for name, method, counts in (("moduli", ca.OZAKI_MODULI, range(8, 21, 2)),
+                                     ("slices", ca.OZAKI_SLICES, range(4, 9))):
+            for cnt in counts:
+                info = {}
+
+                def oz():
+                    info.update(h.dgemm_ozaki("N", "N", n, n, n, 1.0, A.data_ptr(), n, B.data_ptr(), n, 0.0,
+                                              C.data_ptr(), n, method=method, count=cnt))
+                oz()
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                oz()
+                torch.cuda.synchronize()
+                t = time.perf_counter() - t0
+                sweep.append({"method": name, "count": cnt, "bits": info["bits"], "ms": round(t * 1e3, 3),
+                              "rel_err": rows_error(A, B, C, n, rows, ref), "error_estimate": info["error_bound"]})
+        entry["count_sweep"] = sweep
         results.append(entry)
     del A, B, C
     torch.cuda.empty_cache()

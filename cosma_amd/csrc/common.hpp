@@ -108,9 +108,12 @@ struct GroupedProblem {
     int flags;            // bit 0: A transposed, bit 1: B transposed, bit 2: 16-byte operand loads
     int vec_c;            // 16-byte C accesses
 };
+static_assert(sizeof(GroupedProblem) % 8 == 0, "the int64 tile starts follow the descriptor array");
 int grouped_tile_shape(int cfg, int *bm, int *bn);
 int grouped_problem_flags(char ta, char tb, int64_t m, int64_t n, const double *a, int64_t lda, const double *b,
                           int64_t ldb, const double *c, int64_t ldc, int *flags, int *vec_c);
+// probs: nprob descriptors in device memory, sorted as planned, followed by
+// their nprob int64 tile_start values (the kernels search that array).
 int launch_grouped(int cfg, const GroupedProblem *probs, int nprob, int64_t total_tiles, int grid, hipStream_t s);
 int grouped_create(cosma_amd_handle_t h, const cosma_amd_gemm_desc *d, int64_t count, int tile_config,
                    cosma_amd_grouped_t *out);

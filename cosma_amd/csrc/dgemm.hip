@@ -413,16 +413,21 @@ __global__ __launch_bounds__(NT) void dgemm_mfma_kernel(
     gemm_tile<BM, BN, TA, TB, VEC>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, vec_c, m0, n0, sm);
 }
 
-// Grouped tiles without LDS ("direct"): one wave per 64 x 64 C tile (up to
-// 4 x 4 blocks of 16 x 16 in AGPR accumulators; blocks outside C are skipped),
+// Grouped tiles without LDS ("direct"): one wave per (16 FI) x (16 FJ) C tile
+// (64 x 64 or 32 x 32: up to FI x FJ blocks of 16 x 16 in AGPR accumulators;
+// blocks outside C are skipped),
 // fragments loaded straight from global memory (L1 / L2 cached) one 16-deep
 // k-step ahead of their MFMAs; no LDS and no barriers, so small problems pay
 // neither the staging round trip nor the workgroup synchronisation.  The
 // order of the products in every accumulator is gemm_tile's (per 16-deep
 // step: k = 2g, then 2g + 1, then 8 + 2g, then 9 + 2g in lane group g, zero
 // beyond K), the A rows are permuted the same way and the epilogue is the
-// same, so the results are bit-identical to gemm_tile's.
-template <bool TA, bool TB>
+// same, so the results are bit-identical to gemm_tile's.  The 32 x 32 shape
+// pads small problems less (0.77 of the padded MFMA work is useful on the
+// 4096-problem log-uniform batch, against 0.63 for 64 x 64) and, with a
+// quarter of the accumulators, leaves room for several waves per SIMD to hide
+// the load latency.
+template <int FI, int FJ, bool TA, bool TB>
 __device__ __forceinline__ void direct_tile(const GroupedProblem &q, int64_t m0, int64_t n0) {
     const int lane = threadIdx.x & 63, g = lane >> 4, li = lane & 15;
     const int arow = 4 * (li & 3) + (li >> 2);
@@ -430,20 +435,20 @@ __device__ __forceinline__ void direct_tile(const GroupedProblem &q, int64_t m0,
     const bool scale_only = (q.k == 0 || q.alpha == 0.0);
     const int64_t K = scale_only ? 0 : q.k;
     const double alpha = scale_only ? 0.0 : q.alpha;
-    const int rows_in = (int)min((int64_t)4, (M - m0 + 15) / 16);
-    const int cols_in = (int)min((int64_t)4, (N - n0 + 15) / 16);
+    const int rows_in = (int)min((int64_t)FI, (M - m0 + 15) / 16);
+    const int cols_in = (int)min((int64_t)FJ, (N - n0 + 15) / 16);
     const double *__restrict__ A = q.a;
     const double *__restrict__ B = q.b;
-    f64x4 acc[4][4];
+    f64x4 acc[FI][FJ];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < FI; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < FJ; ++j)
             acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(0.0, 0.0, f64x4{0.0, 0.0, 0.0, 0.0}, 0, 0, 0);
     const int koff[4] = {2 * g, 2 * g + 1, 8 + 2 * g, 9 + 2 * g};
-    auto load_step = [&](int64_t k0, double (&a)[4][4], double (&b)[4][4]) {
+    auto load_step = [&](int64_t k0, double (&a)[FI][4], double (&b)[FJ][4]) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < FI; ++i) {
             const int64_t r = m0 + 16 * i + arow;
 #pragma unroll
             for (int x = 0; x < 4; ++x) {
@@ -452,7 +457,7 @@ __device__ __forceinline__ void direct_tile(const GroupedProblem &q, int64_t m0,
             }
         }
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < FJ; ++j) {
             const int64_t c = n0 + 16 * j + li;
 #pragma unroll
             for (int x = 0; x < 4; ++x) {
@@ -461,18 +466,18 @@ __device__ __forceinline__ void direct_tile(const GroupedProblem &q, int64_t m0,
             }
         }
     };
-    auto mfma_step = [&](const double (&a)[4][4], const double (&b)[4][4]) {
+    auto mfma_step = [&](const double (&a)[FI][4], const double (&b)[FJ][4]) {
 #pragma unroll
         for (int x = 0; x < 4; ++x)
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < FI; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
+                for (int j = 0; j < FJ; ++j)
                     if (i < rows_in && j < cols_in)
                         acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][x], b[j][x], acc[i][j], 0, 0, 0);
     };
     const int64_t nsteps = (K + 15) / 16;
-    double a0[4][4], b0[4][4], a1[4][4], b1[4][4];
+    double a0[FI][4], b0[FJ][4], a1[FI][4], b1[FJ][4];
     if (nsteps > 0) load_step(0, a0, b0);
     for (int64_t s = 0; s < nsteps; s += 2) {
         if (s + 1 < nsteps) load_step(16 * (s + 1), a1, b1);
@@ -484,11 +489,11 @@ __device__ __forceinline__ void direct_tile(const GroupedProblem &q, int64_t m0,
     const bool beta0 = (q.beta == 0.0);
     const double beta = q.beta;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < FI; ++i) {
         const int64_t row = m0 + 16 * i + 4 * g;
         if (row >= M) continue;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < FJ; ++j) {
             const int64_t col = n0 + 16 * j + li;
             if (col >= N) continue;
             double *cp = q.c + row + col * q.ldc;
@@ -518,24 +523,50 @@ __device__ __forceinline__ void direct_tile(const GroupedProblem &q, int64_t m0,
     }
 }
 
+// Grouped launches walk virtual tile ids v = blockIdx.x + i * gridDim.x; the
+// hardware places workgroup b on XCD b % 8.  Consecutive tiles (of one
+// problem, which share A and B panels) are kept on one XCD and its L2 by
+// handing out chunks of GROUP_CHUNK consecutive tiles: XCD x takes chunks x,
+// x + 8, x + 16, ... in dispatch order, so the decreasing-cost order of the
+// problems is kept at chunk granularity.  Bijective on [0, T): ids beyond
+// the last whole round of 8 chunks map to themselves.
+constexpr int64_t GROUP_CHUNK = 8;
+__device__ __forceinline__ int64_t grouped_tile_of(int64_t v, int64_t T) {
+    constexpr int64_t span = NXCD * GROUP_CHUNK;
+    if (v >= T / span * span) return v;
+    const int64_t xcd = v % NXCD, local = v / NXCD;
+    return ((local / GROUP_CHUNK) * NXCD + xcd) * GROUP_CHUNK + local % GROUP_CHUNK;
+}
+
+// The descriptor array is followed by the problems' first tile indices
+// (int64, same order): the binary search for a tile's problem walks this
+// compact array (8 bytes per problem, so its upper levels stay in the scalar
+// cache) instead of one 112-byte descriptor per step.
+__device__ __forceinline__ int grouped_owner(const GroupedProblem *probs, int nprob, int64_t t) {
+    const int64_t *starts = reinterpret_cast<const int64_t *>(probs + nprob);
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (starts[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int FI, int FJ>
 __global__ __launch_bounds__(64) void dgemm_grouped_direct(const GroupedProblem *__restrict__ probs, int nprob,
                                                            int64_t total_tiles) {
-    for (int64_t t = blockIdx.x; t < total_tiles; t += gridDim.x) {
-        int lo = 0, hi = nprob - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) / 2;
-            if (probs[mid].tile_start <= t) lo = mid;
-            else hi = mid - 1;
-        }
-        const GroupedProblem &q = probs[lo];
+    for (int64_t v = blockIdx.x; v < total_tiles; v += gridDim.x) {
+        const int64_t t = grouped_tile_of(v, total_tiles);
+        const GroupedProblem &q = probs[grouped_owner(probs, nprob, t)];
         const int64_t local = t - q.tile_start;
-        const int64_t tiles_m = (q.m + 63) / 64;
-        const int64_t m0 = (local % tiles_m) * 64, n0 = (local / tiles_m) * 64;
+        const int64_t tiles_m = (q.m + 16 * FI - 1) / (16 * FI);
+        const int64_t m0 = (local % tiles_m) * (16 * FI), n0 = (local / tiles_m) * (16 * FJ);
         switch (q.flags & 3) {   // bit 0: A transposed, bit 1: B transposed
-        case 0: direct_tile<false, false>(q, m0, n0); break;
-        case 1: direct_tile<true, false>(q, m0, n0); break;
-        case 2: direct_tile<false, true>(q, m0, n0); break;
-        default: direct_tile<true, true>(q, m0, n0); break;
+        case 0: direct_tile<FI, FJ, false, false>(q, m0, n0); break;
+        case 1: direct_tile<FI, FJ, true, false>(q, m0, n0); break;
+        case 2: direct_tile<FI, FJ, false, true>(q, m0, n0); break;
+        default: direct_tile<FI, FJ, true, true>(q, m0, n0); break;
         }
     }
 }
@@ -605,21 +636,18 @@ int launch_cfg(char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha,
 // One persistent launch over the tiles of many independent problems.  Tiles
 // are numbered problem by problem (problems in the order of the descriptor
 // array, which the host sorts by decreasing cost) and column-major inside a
-// problem; workgroup w takes tiles w, w + G, w + 2G, ...  Each tile runs the
+// problem; workgroup w takes the virtual ids w, w + G, w + 2G, ... mapped to
+// tiles by grouped_tile_of (XCD-local chunks).  Each tile runs the
 // same gemm_tile as the single-GEMM kernel, so a problem's result is
 // bit-identical to a cosma_amd_dgemm call on it.
 template <int BM, int BN>
 __global__ __launch_bounds__(NT) void dgemm_grouped_kernel(const GroupedProblem *__restrict__ probs, int nprob,
                                                            int64_t total_tiles) {
     __shared__ TileSmem<BM, BN> sm;
-    for (int64_t t = blockIdx.x; t < total_tiles; t += gridDim.x) {
+    for (int64_t v = blockIdx.x; v < total_tiles; v += gridDim.x) {
+        const int64_t t = grouped_tile_of(v, total_tiles);
         // problem owning tile t: last p with tile_start <= t
-        int lo = 0, hi = nprob - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (probs[mid].tile_start <= t) lo = mid;
-            else hi = mid - 1;
-        }
+        const int lo = grouped_owner(probs, nprob, t);
         const GroupedProblem &q = probs[lo];
         const int64_t local = t - q.tile_start;
         const int64_t tiles_m = (q.m + BM - 1) / BM;
@@ -705,6 +733,7 @@ int grouped_tile_shape(int cfg, int *bm, int *bn) {
     case 2: *bm = 128; *bn = 64; return COSMA_AMD_OK;
     case 3: *bm = 128; *bn = 128; return COSMA_AMD_OK;
     case 4: *bm = 64; *bn = 64; return COSMA_AMD_OK;   // direct: one wave per tile, no LDS
+    case 5: *bm = 32; *bn = 32; return COSMA_AMD_OK;   // direct, 32 x 32
     default:
         set_error("grouped dgemm: unknown tile configuration %d", cfg);
         return COSMA_AMD_ERR_INVALID_ARG;
@@ -728,7 +757,8 @@ int launch_grouped(int cfg, const GroupedProblem *probs, int nprob, int64_t tota
     case 1: hipLaunchKernelGGL((dgemm_grouped_kernel<64, 64>), g, b, 0, s, probs, nprob, total_tiles); break;
     case 2: hipLaunchKernelGGL((dgemm_grouped_kernel<128, 64>), g, b, 0, s, probs, nprob, total_tiles); break;
     case 3: hipLaunchKernelGGL((dgemm_grouped_kernel<128, 128>), g, b, 0, s, probs, nprob, total_tiles); break;
-    case 4: hipLaunchKernelGGL(dgemm_grouped_direct, g, dim3(64), 0, s, probs, nprob, total_tiles); break;
+    case 4: hipLaunchKernelGGL((dgemm_grouped_direct<4, 4>), g, dim3(64), 0, s, probs, nprob, total_tiles); break;
+    case 5: hipLaunchKernelGGL((dgemm_grouped_direct<2, 2>), g, dim3(64), 0, s, probs, nprob, total_tiles); break;
     default:
         set_error("grouped dgemm: unknown tile configuration %d", cfg);
         return COSMA_AMD_ERR_INVALID_ARG;

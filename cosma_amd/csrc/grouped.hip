@@ -12,10 +12,10 @@
 // longest-processing-time order).  Running a plan is a single kernel launch
 // with no allocation, no memset and no host synchronisation, so it can be
 // captured; cosma_amd_grouped_graph_run captures it once into a hipGraph and
-// replays that.  The tuner times every tile shape (64x64, 128x64, 128x128)
-// with persistent grids of 1, 2, 4 and 8 workgroups per CU and one workgroup
-// per tile, keeps the fastest, and remembers it process-wide for plans with
-// the same problem shapes.
+// replays that.  The tuner times every tile configuration (LDS-staged 64x64,
+// 128x64, 128x128; direct 64x64 and 32x32) with persistent grids of 1, 2, 4
+// and 8 workgroups per CU and one workgroup per tile, keeps the fastest, and
+// remembers it process-wide for plans with the same problem shapes.
 #include <algorithm>
 #include <map>
 #include <mutex>
@@ -28,10 +28,10 @@ struct cosma_amd_grouped_s {
     cosma_amd_handle_t h = nullptr;
     int nprob = 0;
     uint64_t signature = 0;
-    // per tile configuration 1..4; configuration 4 (64 x 64, direct) shares
+    // per tile configuration 1..5; configuration 4 (64 x 64, direct) shares
     // configuration 1's descriptors
-    cosma_amd::GroupedProblem *dev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-    int64_t total_tiles[5] = {0, 0, 0, 0, 0};
+    cosma_amd::GroupedProblem *dev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    int64_t total_tiles[6] = {0, 0, 0, 0, 0, 0};
     int cfg = 1;
     int grid = 1;
     hipStream_t capture = nullptr;
@@ -84,8 +84,8 @@ int grouped_create(cosma_amd_handle_t h, const cosma_amd_gemm_desc *d, int64_t c
         set_error("grouped dgemm: invalid problem list");
         return COSMA_AMD_ERR_INVALID_ARG;
     }
-    if (tile_config < 0 || tile_config > 4) {
-        set_error("grouped dgemm: tile_config %d is not 0..4", tile_config);
+    if (tile_config < 0 || tile_config > 5) {
+        set_error("grouped dgemm: tile_config %d is not 0..5", tile_config);
         return COSMA_AMD_ERR_INVALID_ARG;
     }
     std::vector<GroupedProblem> probs;
@@ -134,7 +134,8 @@ int grouped_create(cosma_amd_handle_t h, const cosma_amd_gemm_desc *d, int64_t c
     p->nprob = (int)probs.size();
     p->signature = fnv(sig, (uint64_t)probs.size());
     int rc = COSMA_AMD_OK;
-    for (int cfg = 1; cfg <= 3 && rc == COSMA_AMD_OK; ++cfg) {
+    for (int cfg : {1, 2, 3, 5}) {
+        if (rc != COSMA_AMD_OK) break;
         int bm, bn;
         grouped_tile_shape(cfg, &bm, &bn);
         std::vector<GroupedProblem> sorted;
@@ -148,14 +149,20 @@ int grouped_create(cosma_amd_handle_t h, const cosma_amd_gemm_desc *d, int64_t c
         }
         p->total_tiles[cfg] = t;
         if (sorted.empty()) continue;
-        hipError_t e = hipMalloc(&p->dev[cfg], sorted.size() * sizeof(GroupedProblem));
+        // descriptors, then their first tile indices (the kernels' search array)
+        std::vector<int64_t> starts;
+        for (const GroupedProblem &g : sorted) starts.push_back(g.tile_start);
+        const size_t dbytes = sorted.size() * sizeof(GroupedProblem), sbytes = starts.size() * sizeof(int64_t);
+        hipError_t e = hipMalloc(&p->dev[cfg], dbytes + sbytes);
         if (e != hipSuccess) {
             (void)hipGetLastError();
             set_error("grouped dgemm: descriptor allocation failed (%s)", hipGetErrorString(e));
             rc = COSMA_AMD_ERR_ALLOC;
             break;
         }
-        e = hipMemcpy(p->dev[cfg], sorted.data(), sorted.size() * sizeof(GroupedProblem), hipMemcpyHostToDevice);
+        e = hipMemcpy(p->dev[cfg], sorted.data(), dbytes, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(reinterpret_cast<char *>(p->dev[cfg]) + dbytes, starts.data(), sbytes, hipMemcpyHostToDevice);
         if (e != hipSuccess) {
             set_error("grouped dgemm: descriptor upload failed (%s)", hipGetErrorString(e));
             rc = COSMA_AMD_ERR_HIP;
@@ -217,7 +224,7 @@ int grouped_tune(cosma_amd_grouped_t p, hipStream_t s) {
     float best = -1.0f;
     int best_cfg = p->cfg, best_grid = p->grid;
     int rc = COSMA_AMD_OK;
-    for (int cfg = 1; cfg <= 4 && rc == COSMA_AMD_OK; ++cfg) {
+    for (int cfg = 1; cfg <= 5 && rc == COSMA_AMD_OK; ++cfg) {
         const int64_t tiles = p->total_tiles[cfg];
         std::vector<int64_t> grids;
         for (int per_cu : {1, 2, 4, 8}) grids.push_back(std::min<int64_t>(tiles, (int64_t)p->h->n_cu * per_cu));
@@ -252,8 +259,8 @@ int grouped_tune(cosma_amd_grouped_t p, hipStream_t s) {
 }
 
 int grouped_set_config(cosma_amd_grouped_t p, int tile_config, int grid) {
-    if (tile_config < 1 || tile_config > 4) {
-        set_error("grouped dgemm: tile_config %d is not 1..4", tile_config);
+    if (tile_config < 1 || tile_config > 5) {
+        set_error("grouped dgemm: tile_config %d is not 1..5", tile_config);
         return COSMA_AMD_ERR_INVALID_ARG;
     }
     const int g = grid > 0 ? (int)std::min<int64_t>(grid, std::max<int64_t>(1, p->total_tiles[tile_config]))
@@ -275,7 +282,7 @@ void grouped_destroy(cosma_amd_grouped_t p) {
     hipSetDevice(p->h->device);
     drop_graph(p);
     if (p->capture) hipStreamDestroy(p->capture);
-    for (int cfg = 1; cfg <= 3; ++cfg)   // dev[4] aliases dev[1]
+    for (int cfg : {1, 2, 3, 5})   // dev[4] aliases dev[1]
         if (p->dev[cfg]) hipFree(p->dev[cfg]);
     delete p;
 }

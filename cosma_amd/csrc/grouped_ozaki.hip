@@ -57,6 +57,8 @@ struct OzProblem {
     int64_t tile_start;      // first INT8 tile of the problem in the product launch
 };
 
+static_assert(sizeof(OzProblem) % 8 == 0, "the int64 tile starts follow the descriptor array");
+
 struct GzModuli {
     int count;
     int m[MAX_MODULI];
@@ -257,11 +259,14 @@ __global__ __launch_bounds__(256) void gz_products(const OzProblem *probs, int n
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = wave & 1, wn = wave >> 1, g = lane >> 4, li = lane & 15;
     const int lrow = t >> 2, lchunk = t & 3;   // this thread's 16 bytes of a 64 x 64-byte step
+    // the descriptors are followed by their int64 tile_start values: the
+    // search walks that compact array
+    const int64_t *starts = reinterpret_cast<const int64_t *>(probs + nprob);
     for (int64_t item = blockIdx.x; item < total; item += gridDim.x) {
         int lo = 0, hi = nprob - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) / 2;
-            if (probs[mid].tile_start <= item) lo = mid;
+            if (starts[mid] <= item) lo = mid;
             else hi = mid - 1;
         }
         const OzProblem &q = probs[lo];
@@ -587,8 +592,10 @@ int grouped_ozaki_create(cosma_amd_handle_t h, const cosma_amd_gemm_desc *d, int
         return COSMA_AMD_OK;
     };
     int rc;
-    const size_t ndesc = sorted.size() * sizeof(OzProblem);
-    if ((rc = alloc(reinterpret_cast<void **>(&p->dev), ndesc))) return fail(rc);
+    // descriptors, then their int64 first tile indices (the product kernel's
+    // search array)
+    const size_t ndesc = sorted.size() * sizeof(OzProblem), nstart = sorted.size() * sizeof(int64_t);
+    if ((rc = alloc(reinterpret_cast<void **>(&p->dev), ndesc + nstart))) return fail(rc);
     if ((rc = alloc(reinterpret_cast<void **>(&p->flags), sorted.size() * sizeof(int)))) return fail(rc);
 
     // exponents first (their space does not depend on the moduli count)
@@ -605,7 +612,10 @@ int grouped_ozaki_create(cosma_amd_handle_t h, const cosma_amd_gemm_desc *d, int
         }
     }
     auto upload = [&]() -> int {
+        std::vector<int64_t> starts;
+        for (const auto &g : sorted) starts.push_back(g.tile_start);
         CA_HIP(hipMemcpy(p->dev, sorted.data(), ndesc, hipMemcpyHostToDevice));
+        CA_HIP(hipMemcpy(reinterpret_cast<char *>(p->dev) + ndesc, starts.data(), nstart, hipMemcpyHostToDevice));
         return COSMA_AMD_OK;
     };
 

@@ -229,8 +229,8 @@ typedef struct cosma_amd_grouped_s *cosma_amd_grouped_t;
  * pointers must stay valid while the plan is used.  tile_config: 0 = the
  * tuned choice for these shapes if one exists in the process, else 64x64;
  * 1 = 64x64, 2 = 128x64, 3 = 128x128 C tiles staged through LDS by four
- * waves; 4 = 64x64 C tiles computed by one wave each with fragments loaded
- * straight from global memory (no LDS).
+ * waves; 4 = 64x64 and 5 = 32x32 C tiles computed by one wave each with
+ * fragments loaded straight from global memory (no LDS).
  * cosma_amd_grouped_run: one persistent kernel launch on `stream` (no
  * allocation, memset or synchronisation: capturable).
  * cosma_amd_grouped_graph_run: captures the launch into a hipGraph on first
@@ -248,7 +248,7 @@ int cosma_amd_grouped_create(cosma_amd_handle_t handle, const cosma_amd_gemm_des
 int cosma_amd_grouped_run(cosma_amd_grouped_t plan, void *stream);
 int cosma_amd_grouped_graph_run(cosma_amd_grouped_t plan, void *stream);
 int cosma_amd_grouped_tune(cosma_amd_grouped_t plan, void *stream);
-/* Forces a tile configuration (1..4) and grid (<= 0: default) for the plan. */
+/* Forces a tile configuration (1..5) and grid (<= 0: default) for the plan. */
 int cosma_amd_grouped_set_config(cosma_amd_grouped_t plan, int tile_config, int grid);
 /* Current tile configuration, grid and number of tiles of the plan. */
 int cosma_amd_grouped_info(cosma_amd_grouped_t plan, int *tile_config, int *grid, int64_t *tiles);

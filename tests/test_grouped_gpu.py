@@ -83,7 +83,7 @@ def problems(handle):
     return probs, single_results(handle, probs)
 
 
-@pytest.mark.parametrize("cfg", [1, 2, 3, 4])
+@pytest.mark.parametrize("cfg", [1, 2, 3, 4, 5])
 def test_grouped_run_bit_identical_to_single_calls(handle, problems, cfg):
     import torch
     import cosma_amd as ca
@@ -141,7 +141,7 @@ def test_tuner_picks_a_valid_configuration(handle, problems):
     plan = ca.Grouped(handle, descs(probs))
     plan.tune()
     info = plan.info()
-    assert info["tile_config"] in (1, 2, 3, 4) and 1 <= info["grid"] <= max(1, info["tiles"])
+    assert info["tile_config"] in (1, 2, 3, 4, 5) and 1 <= info["grid"] <= max(1, info["tiles"])
     # a second plan over the same shapes starts from the tuned choice
     plan2 = ca.Grouped(handle, descs(probs))
     assert plan2.info()["tile_config"] == info["tile_config"] and plan2.info()["grid"] == info["grid"]
