@@ -14,7 +14,7 @@
 //     in a "k-pair" layout  tile[kp][row] = (x[row][2kp], x[row][2kp+1])  so
 //     that one 16-byte LDS read per lane feeds two v_mfma_f64_16x16x4f64
 //     (lane group g = lane/16 supplies k = 2kp and 2kp+1 of the same pair).
-//     Rows are stored XOR-swizzled, tile[kp][row ^ kp] (kp < 8 flips the low
+//     Rows are stored XOR-swizzled, tile[kp][row ^ kp ^ (bit 4 of row)] (flips the low
 //     three row bits), with no padding: the MFMA reads (16 lanes on 16
 //     consecutive rows of one kp) and the k-contiguous stores (eight lanes
 //     per row, one per kp) both hit distinct bank groups.  Measured with
@@ -53,6 +53,14 @@ constexpr int BK = 16;
 constexpr int KP = BK / 2;   // k-pairs per step
 constexpr int NT = 256;      // threads per workgroup
 constexpr int NXCD = 8;      // XCDs on MI355X
+
+// LDS position of row `row` in k-pair plane kp: the row XOR (kp ^ bit 4 of
+// the row).  The flip stays inside an aligned group of 8 rows, so the 16
+// consecutive rows of a fragment read land on 16 different 16-byte bank
+// groups; bit 4 separates the two halves of the 32 rows that 16 lanes store
+// at once for a row-contiguous operand (rows 2l and 2l + 1 per lane), which
+// with kp alone met in pairs (a 2-way conflict on every such store).
+__device__ __forceinline__ int lds_swz(int row, int kp) { return row ^ kp ^ ((row >> 4) & 1); }
 
 // Loads of one operand tile (ROWS x BK) into registers.
 //   RowContig: element (r, k) at x[r + k * ld]   (A 'N', B 'T')
@@ -162,12 +170,12 @@ struct TileIO {
                 f64x2 lo, hi;
                 lo.x = v[2 * q].x;  lo.y = v[2 * q + 1].x;   // row r:   (k, k+1)
                 hi.x = v[2 * q].y;  hi.y = v[2 * q + 1].y;   // row r+1: (k, k+1)
-                tile[kp][r ^ kp] = lo;
-                tile[kp][(r + 1) ^ kp] = hi;
+                tile[kp][lds_swz(r, kp)] = lo;
+                tile[kp][lds_swz(r + 1, kp)] = hi;
             } else {
                 int row, kp;
                 kmap(idx, row, kp);
-                tile[kp][row ^ kp] = v[q];
+                tile[kp][lds_swz(row, kp)] = v[q];
             }
         }
     }
@@ -236,9 +244,9 @@ __device__ __forceinline__ void gemm_tile(
     auto read_frags = [&](int st, int ks, int h) {
         const int kp = 4 * ks + g;
 #pragma unroll
-        for (int i = 0; i < FM; ++i) af[h][i] = sa[st][kp][(abase + 16 * i) ^ kp];
+        for (int i = 0; i < FM; ++i) af[h][i] = sa[st][kp][lds_swz(abase + 16 * i, kp)];
 #pragma unroll
-        for (int j = 0; j < FN; ++j) bf[h][j] = sb[st][kp][(bbase + 16 * j) ^ kp];
+        for (int j = 0; j < FN; ++j) bf[h][j] = sb[st][kp][lds_swz(bbase + 16 * j, kp)];
     };
     // blocks of this wave that hold rows / columns of C (all of them unless EDGE)
     const int64_t wrow0 = m0 + wm * (BM / 2), wcol0 = n0 + wn * (BN / 2);
