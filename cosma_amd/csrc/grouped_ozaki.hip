@@ -75,6 +75,7 @@ struct cosma_amd_grouped_ozaki_s {
     double estimate = 0.0;
     int64_t total_tiles = 0;
     int grid = 1;
+    int vgroups = 1;   // max over the problems of max(Mp, Np) / 64
     cosma_amd::GzModuli md{};
     cosma_amd::CrtConst cc{};
     cosma_amd::OzProblem *dev = nullptr;   // descriptors, sorted by decreasing cost
@@ -99,59 +100,96 @@ __device__ __forceinline__ double vec_at(const double *x, int64_t ld, bool conti
     return contig ? x[l + r * ld] : x[r + l * ld];
 }
 
-// Exponents of the R vectors (rows of op(A) or columns of op(B)); entries
-// [R, Rp) get 0.  A strided operand is read with one thread per vector (a
-// wave reads 64 consecutive elements per load); a contiguous one with one
-// wave per vector and a lane reduction.
-__device__ void vector_exponents(const double *x, int64_t ld, bool contig, int64_t R, int64_t K, int64_t Rp,
-                                 int *Eout, int *bad) {
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    if (!contig) {
-        for (int64_t r = t; r < Rp; r += blockDim.x) {
-            unsigned long long best = 0;
-            if (r < R)
-                for (int64_t l = 0; l < K; ++l) {
-                    const unsigned long long b = __double_as_longlong(fabs(x[r + l * ld]));
-                    best = b > best ? b : best;
-                }
-            if (best >= 0x7ff0000000000000ull) atomicOr(bad, 1);
-            Eout[r] = r < R ? row_exponent(best, 0) : 0;
-        }
-    } else {
-        const int waves = blockDim.x >> 6;
-        for (int64_t r = wave; r < Rp; r += waves) {
-            unsigned long long best = 0;
-            if (r < R)
-                for (int64_t l = lane; l < K; l += 64) {
-                    const unsigned long long b = __double_as_longlong(fabs(x[l + r * ld]));
-                    best = b > best ? b : best;
-                }
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                const unsigned long long o = __shfl_xor(best, off);
-                best = o > best ? o : best;
-            }
-            if (lane == 0) {
-                if (best >= 0x7ff0000000000000ull) atomicOr(bad, 1);
-                Eout[r] = r < R ? row_exponent(best, 0) : 0;
-            }
-        }
-    }
+// Exponents of the rows of op(A) and the columns of op(B), and each problem's
+// non-finite flag (which gz_clear_flags zeroes first).  Workgroup (p, y):
+// problem p, operand y & 1 (0: op(A) rows, 1: op(B) columns), vectors
+// [64 (y >> 1), 64 (y >> 1) + 64); entries in [R, Rp) get exponent 0.  A
+// strided operand (vector r at x[r + l * ld]) is read by lane = vector, the
+// four waves taking every fourth l with eight loads in flight; a contiguous
+// one (x[l + r * ld]) by lanes along l, a wave taking 16 of the vectors four
+// at a time, and reduced across the lanes.
+__global__ void gz_clear_flags(int *flags, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) flags[i] = 0;
 }
 
-// One workgroup per problem: exponents of the rows of op(A) and the columns
-// of op(B), and the problem's non-finite flag.
 __global__ __launch_bounds__(256) void gz_exponents(const OzProblem *probs, int *flags) {
     const OzProblem &q = probs[blockIdx.x];
-    __shared__ int bad;
-    if (threadIdx.x == 0) bad = 0;
-    __syncthreads();
-    if (!q.scale_only) {
-        vector_exponents(q.a, q.lda, q.ta != 0, q.m, q.k, q.Mp, q.E, &bad);
-        vector_exponents(q.b, q.ldb, q.tb == 0, q.n, q.k, q.Np, q.F, &bad);
+    if (q.scale_only) return;
+    const bool opA = (blockIdx.y & 1) == 0;
+    const int64_t r0 = (int64_t)(blockIdx.y >> 1) * 64;
+    const int64_t R = opA ? q.m : q.n, Rp = opA ? q.Mp : q.Np, K = q.k;
+    if (r0 >= Rp) return;
+    const double *x = opA ? q.a : q.b;
+    const int64_t ld = opA ? q.lda : q.ldb;
+    const bool contig = opA ? q.ta != 0 : q.tb == 0;
+    __shared__ unsigned long long part[4][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    auto bits = [](double v) { return (unsigned long long)__double_as_longlong(fabs(v)); };
+    if (!contig) {
+        const int64_t r = r0 + lane;
+        unsigned long long best = 0;
+        if (r < R) {
+            int64_t l = wave;
+            for (; l + 28 < K; l += 32) {
+                unsigned long long b[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) b[u] = bits(x[r + (l + 4 * u) * ld]);
+#pragma unroll
+                for (int u = 0; u < 8; ++u) best = b[u] > best ? b[u] : best;
+            }
+            for (; l < K; l += 4) {
+                const unsigned long long b = bits(x[r + l * ld]);
+                best = b > best ? b : best;
+            }
+        }
+        part[wave][lane] = best;
+    } else {
+        for (int v0 = wave; v0 < 64; v0 += 16) {
+            unsigned long long b[4] = {0, 0, 0, 0};
+            for (int64_t l = lane; l < K; l += 64) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int64_t r = r0 + v0 + 4 * u;
+                    if (r < R) {
+                        const unsigned long long c = bits(x[l + r * ld]);
+                        b[u] = c > b[u] ? c : b[u];
+                    }
+                }
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const unsigned long long o = __shfl_xor(b[u], off);
+                    b[u] = o > b[u] ? o : b[u];
+                }
+            }
+            if (lane < 4) {
+                unsigned long long mine = b[0];
+#pragma unroll
+                for (int u = 1; u < 4; ++u) mine = lane == u ? b[u] : mine;
+                const int v = v0 + 4 * lane;
+                part[v & 3][v >> 2] = mine;
+            }
+        }
     }
     __syncthreads();
-    if (threadIdx.x == 0) flags[blockIdx.x] = bad;
+    if (threadIdx.x < 64) {
+        const int v = threadIdx.x;
+        const int64_t r = r0 + v;
+        if (r < Rp) {
+            unsigned long long b;
+            if (!contig) {
+                b = part[0][v];
+#pragma unroll
+                for (int w = 1; w < 4; ++w) b = part[w][v] > b ? part[w][v] : b;
+            } else {
+                b = part[v & 3][v >> 2];
+            }
+            if (r < R && b >= 0x7ff0000000000000ull) atomicOr(&flags[blockIdx.x], 1);   // inf or NaN
+            (opA ? q.E : q.F)[r] = r < R ? row_exponent(b, 0) : 0;
+        }
+    }
 }
 
 // Plan creation only: the error factor gamma_p of each problem (ozaki.hip's
@@ -217,34 +255,67 @@ __global__ __launch_bounds__(256) void gz_gamma(const OzProblem *probs, const in
     }
 }
 
-// Residues of every problem's op(A) rows (blockIdx.y = 0) or op(B) columns
-// (blockIdx.y = 1), padding rows and k with zeros.  A thread takes 16
-// consecutive k of one vector; for a strided operand consecutive threads take
-// consecutive vectors, so each of the 16 loads is one run across the wave.
+// Residues of the rows of op(A) and the columns of op(B), padding rows and k
+// with zeros; workgroup (p, y) as in gz_exponents (problem p, operand y & 1,
+// vectors [64 (y >> 1), 64 (y >> 1) + 64)).  Contiguous operand: a thread
+// takes 16 consecutive k of one vector and consecutive threads continue
+// along the vector, so the residue stores are contiguous runs.  Strided
+// operand: 64 x 64 (vector x k) tiles are read along the vectors into LDS and
+// transposed, four threads per vector writing 64 contiguous bytes per plane.
 __global__ __launch_bounds__(256) void gz_split(const OzProblem *probs, GzModuli md) {
     const OzProblem &q = probs[blockIdx.x];
     if (q.scale_only) return;
-    const bool opA = blockIdx.y == 0;
+    const bool opA = (blockIdx.y & 1) == 0;
+    const int64_t r0 = (int64_t)(blockIdx.y >> 1) * 64;
+    const int64_t R = opA ? q.m : q.n, Rp = opA ? q.Mp : q.Np;
+    if (r0 >= Rp) return;
     const double *x = opA ? q.a : q.b;
     const int64_t ld = opA ? q.lda : q.ldb;
     const bool contig = opA ? q.ta != 0 : q.tb == 0;
-    const int64_t R = opA ? q.m : q.n, Rp = opA ? q.Mp : q.Np;
     const int *ex = opA ? q.E : q.F;
     int8_t *dst = opA ? q.ares : q.bres;
-    const int64_t plane = Rp * q.Kp, nkc = q.Kp / 16, chunks = Rp * nkc;
-    for (int64_t c = threadIdx.x; c < chunks; c += blockDim.x) {
-        const int64_t r = contig ? c / nkc : c % Rp;
-        const int64_t k0 = 16 * (contig ? c % nkc : c / Rp);
-        const bool rin = r < R;
-        if (!rin || k0 >= q.k) {   // padding: residues of zero
-            for (int l = 0; l < md.count; ++l)
-                *reinterpret_cast<i32x4 *>(dst + r * q.Kp + k0 + (int64_t)l * plane) = i32x4{0, 0, 0, 0};
+    const int64_t plane = Rp * q.Kp, nkc = q.Kp / 16;
+    auto zeros = [&](int64_t r, int64_t k0) {
+        for (int l = 0; l < md.count; ++l)
+            *reinterpret_cast<i32x4 *>(dst + r * q.Kp + k0 + (int64_t)l * plane) = i32x4{0, 0, 0, 0};
+    };
+    if (contig) {
+        const int64_t chunks = 64 * nkc;
+        for (int64_t c = threadIdx.x; c < chunks; c += blockDim.x) {
+            const int64_t r = r0 + c / nkc, k0 = 16 * (c % nkc);
+            if (r >= R || k0 >= q.k) {   // padding: residues of zero
+                zeros(r, k0);
+                continue;
+            }
+            double v[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) v[j] = k0 + j < q.k ? x[k0 + j + r * ld] : 0.0;
+            moduli_residues16(v, ex[r], q.P, md.count, md.m, md.inv_m, dst + r * q.Kp + k0, plane);
+        }
+        return;
+    }
+    __shared__ double tile[64][65];
+    const int t = threadIdx.x, tx = t & 63, ty = t >> 6;
+    const int row = t >> 2, kc = t & 3;
+    const int64_t r = r0 + row;
+    const int e = ex[r];
+    for (int64_t k0 = 0; k0 < q.Kp; k0 += 64) {
+        if (k0 >= q.k || r0 >= R) {      // whole tile of padding (uniform branch)
+            zeros(r, k0 + 16 * kc);
             continue;
         }
+        __syncthreads();   // the previous tile's reads are done
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int kk = ty + 4 * u;
+            const int64_t rr = r0 + tx, k = k0 + kk;
+            tile[kk][tx] = (rr < R && k < q.k) ? x[rr + k * ld] : 0.0;
+        }
+        __syncthreads();
         double v[16];
 #pragma unroll
-        for (int j = 0; j < 16; ++j) v[j] = k0 + j < q.k ? vec_at(x, ld, contig, r, k0 + j) : 0.0;
-        moduli_residues16(v, ex[r], q.P, md.count, md.m, md.inv_m, dst + r * q.Kp + k0, plane);
+        for (int j = 0; j < 16; ++j) v[j] = tile[16 * kc + j][row];
+        moduli_residues16(v, r < R ? e : 0, q.P, md.count, md.m, md.inv_m, dst + r * q.Kp + k0 + 16 * kc, plane);
     }
 }
 
@@ -432,8 +503,10 @@ int grouped_ozaki_destroy(cosma_amd_grouped_ozaki_t p) {
 int grouped_ozaki_run(cosma_amd_grouped_ozaki_t p, hipStream_t s) {
     if (p->nprob == 0) return COSMA_AMD_OK;
     CA_HIP(hipSetDevice(p->h->device));
-    hipLaunchKernelGGL(gz_exponents, dim3((unsigned)p->nprob), dim3(256), 0, s, p->dev, p->flags);
-    hipLaunchKernelGGL(gz_split, dim3((unsigned)p->nprob, 2), dim3(256), 0, s, p->dev, p->md);
+    const dim3 gv((unsigned)p->nprob, (unsigned)(2 * p->vgroups));
+    hipLaunchKernelGGL(gz_clear_flags, dim3((unsigned)((p->nprob + 255) / 256)), dim3(256), 0, s, p->flags, p->nprob);
+    hipLaunchKernelGGL(gz_exponents, gv, dim3(256), 0, s, p->dev, p->flags);
+    hipLaunchKernelGGL(gz_split, gv, dim3(256), 0, s, p->dev, p->md);
     if (p->total_tiles > 0)
         hipLaunchKernelGGL(gz_products, dim3((unsigned)p->grid), dim3(256), 0, s, p->dev, p->nprob, p->total_tiles,
                            p->md);
@@ -581,6 +654,15 @@ int grouped_ozaki_create(cosma_amd_handle_t h, const cosma_amd_gemm_desc *d, int
         grouped_ozaki_destroy(p);
         return rc;
     };
+    // vector groups of 64 per operand: the y extent of the exponent and split grids
+    int64_t vg = 1;
+    for (const auto &g : sorted) vg = std::max(vg, std::max(g.Mp, g.Np) / 64);
+    if (2 * vg > 65535) {
+        set_error("grouped ozaki: a problem with %lld rows or columns exceeds the launch limit",
+                  (long long)(64 * vg));
+        return fail(COSMA_AMD_ERR_UNSUPPORTED);
+    }
+    p->vgroups = (int)vg;
     auto alloc = [&](void **ptr, size_t bytes) -> int {
         const hipError_t e = hipMalloc(ptr, std::max<size_t>(bytes, 256));
         if (e != hipSuccess) {
@@ -626,7 +708,10 @@ int grouped_ozaki_create(cosma_amd_handle_t h, const cosma_amd_gemm_desc *d, int
     {
         double *dgamma = nullptr;
         if ((rc = alloc(reinterpret_cast<void **>(&dgamma), sorted.size() * sizeof(double)))) return fail(rc);
-        hipLaunchKernelGGL(gz_exponents, dim3((unsigned)p->nprob), dim3(256), 0, 0, p->dev, p->flags);
+        const dim3 gv((unsigned)p->nprob, (unsigned)(2 * p->vgroups));
+        hipLaunchKernelGGL(gz_clear_flags, dim3((unsigned)((p->nprob + 255) / 256)), dim3(256), 0, 0, p->flags,
+                           p->nprob);
+        hipLaunchKernelGGL(gz_exponents, gv, dim3(256), 0, 0, p->dev, p->flags);
         hipLaunchKernelGGL(gz_gamma, dim3((unsigned)p->nprob), dim3(256), 0, 0, p->dev, p->flags, dgamma);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) e = hipMemcpy(gamma.data(), dgamma, gamma.size() * sizeof(double), hipMemcpyDeviceToHost);
