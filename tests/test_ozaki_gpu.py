@@ -268,24 +268,26 @@ def test_default_tolerance_holds_for_spread_exponents(handle):
     assert info["count"] > info_u["count"], (info, info_u)
 
 
-def test_nonfinite_inputs_fall_back_to_fp64(handle):
+@pytest.mark.parametrize("method", [0, 2])
+def test_nonfinite_inputs_fall_back_to_fp64(handle, method):
     m, n, k = 40, 30, 50
     A, B = oracle.uniform(m, k, 1), oracle.uniform(k, n, 2)
     A[3, 4] = np.inf
-    got, info = run_ozaki(handle, "N", "N", m, n, k, 1.0, A, B, 0.0, np.zeros((m, n), order="F"), 0)
+    got, info = run_ozaki(handle, "N", "N", m, n, k, 1.0, A, B, 0.0, np.zeros((m, n), order="F"), method)
     assert info["method"] == -1
     assert np.isinf(got[3]).any() or np.isnan(got[3]).any()
     assert np.isfinite(np.delete(got, 3, axis=0)).all()
 
 
-def test_semantics_beta_and_degenerate(handle):
+@pytest.mark.parametrize("method", [0, 1, 2])
+def test_semantics_beta_and_degenerate(handle, method):
     m, n, k = 50, 40, 64
     A, B, C0 = oracle.uniform(m, k, 1), oracle.uniform(k, n, 2), oracle.uniform(m, n, 3)
-    got, _ = run_ozaki(handle, "N", "N", m, n, k, 1.0, A, B, 0.0, C0, 0, nan_c=True)
+    got, _ = run_ozaki(handle, "N", "N", m, n, k, 1.0, A, B, 0.0, C0, method, nan_c=True)
     assert np.isfinite(got).all()
-    got, _ = run_ozaki(handle, "N", "N", m, n, 0, 1.0, A, B, 0.25, C0, 0)
+    got, _ = run_ozaki(handle, "N", "N", m, n, 0, 1.0, A, B, 0.25, C0, method)
     assert np.array_equal(got, 0.25 * C0)
-    got, _ = run_ozaki(handle, "N", "N", m, n, k, 0.0, A, B, 2.0, C0, 1)
+    got, _ = run_ozaki(handle, "N", "N", m, n, k, 0.0, A, B, 2.0, C0, method)
     assert np.array_equal(got, 2.0 * C0)
 
 
