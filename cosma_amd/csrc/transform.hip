@@ -16,7 +16,9 @@
 //           (rows of the tile are read along src columns and written along
 //           dst columns); the pad spreads the column reads over all banks.
 //           With aligned pointers and even leading dimensions the global
-//           accesses are 16 bytes wide (two consecutive elements).
+//           accesses are 16 bytes wide (two consecutive elements), and a
+//           workgroup takes several tiles of a column, loading the next one
+//           while it writes the current one (16384^2: 0.865 -> 0.835 ms).
 // alpha == 1 and beta == 0 copies values bit-for-bit (no arithmetic at all);
 // beta == 0 never reads dst.
 #include "common.hpp"
@@ -101,15 +103,24 @@ __global__ __launch_bounds__(TNT) void transpose_vec_kernel(int64_t rows, int64_
     __shared__ double tile[TT][TT + 1];
     const int t = threadIdx.x;
     const int64_t i0 = (int64_t)blockIdx.x * TT;   // dst rows / src cols
-    for (int64_t j0 = (int64_t)blockIdx.y * TT; j0 < cols; j0 += (int64_t)gridDim.y * TT) {
-        const bool full = (i0 + TT <= rows) && (j0 + TT <= cols);
-        if (full) {
-            f64x2 v[8];
+    const int64_t jstep = (int64_t)gridDim.y * TT;
+    auto full = [&](int64_t j0) { return (i0 + TT <= rows) && (j0 + TT <= cols); };
+    // src column i0 + il, rows j0 + c2, c2 + 1
+    auto load = [&](int64_t j0, f64x2 (&v)[8]) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {     // src column i0 + il, rows j0 + c2, c2 + 1
-                const int idx = t + TNT * q, il = idx >> 5, c2 = (idx & 31) * 2;
-                v[q] = *reinterpret_cast<const f64x2 *>(src + (j0 + c2) + (i0 + il) * lds);
-            }
+        for (int q = 0; q < 8; ++q) {
+            const int idx = t + TNT * q, il = idx >> 5, c2 = (idx & 31) * 2;
+            v[q] = *reinterpret_cast<const f64x2 *>(src + (j0 + c2) + (i0 + il) * lds);
+        }
+    };
+    // A workgroup walks the tiles j0, j0 + jstep, ... of its column of
+    // tiles; the loads of the next full tile are issued right after the
+    // current one is in LDS, so they are in flight during its stores.
+    f64x2 v[8];
+    int64_t j0 = (int64_t)blockIdx.y * TT;
+    if (j0 < cols && full(j0)) load(j0, v);
+    for (; j0 < cols; j0 += jstep) {
+        if (full(j0)) {
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
                 const int idx = t + TNT * q, il = idx >> 5, c2 = (idx & 31) * 2;
@@ -117,6 +128,7 @@ __global__ __launch_bounds__(TNT) void transpose_vec_kernel(int64_t rows, int64_
                 tile[il][c2 + 1] = v[q].y;
             }
             __syncthreads();
+            if (j0 + jstep < cols && full(j0 + jstep)) load(j0 + jstep, v);
 #pragma unroll
             for (int q = 0; q < 8; ++q) {     // dst column j0 + jl, rows i0 + r2, r2 + 1
                 const int idx = t + TNT * q, jl = idx >> 5, r2 = (idx & 31) * 2;
@@ -144,6 +156,7 @@ __global__ __launch_bounds__(TNT) void transpose_vec_kernel(int64_t rows, int64_
                 if (i < rows && j < cols) tile[il][tx] = src[j + i * lds];
             }
             __syncthreads();
+            if (j0 + jstep < cols && full(j0 + jstep)) load(j0 + jstep, v);
 #pragma unroll
             for (int r = 0; r < TT / 4; ++r) {
                 const int jl = ty + 4 * r;
@@ -187,6 +200,9 @@ int launch_block_transform(char op, int64_t rows, int64_t cols, double alpha,
             return COSMA_AMD_ERR_UNSUPPORTED;
         }
         const bool vec = aligned16(src) && aligned16(dst) && (lds % 2 == 0) && (ldd % 2 == 0);
+        // with plenty of workgroups, four tiles each (the vectorised kernel
+        // prefetches the next tile of a workgroup during the current one)
+        if (vec && bx * by > 4096) by = (by + 3) / 4;
         if (vec)
             hipLaunchKernelGGL(transpose_vec_kernel, dim3((unsigned)bx, (unsigned)by), dim3(TNT), 0, s,
                                rows, cols, alpha, src, lds, beta, dst, ldd, mode);
