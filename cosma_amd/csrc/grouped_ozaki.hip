@@ -28,6 +28,7 @@
 // computed by plain FP64 dot products in the recombination kernel, so IEEE
 // propagation is kept.
 #include <algorithm>
+#include <cstring>
 #include <cmath>
 #include <numeric>
 #include <vector>
@@ -80,6 +81,9 @@ struct cosma_amd_grouped_ozaki_s {
     cosma_amd::CrtConst cc{};
     cosma_amd::OzProblem *dev = nullptr;   // descriptors, sorted by decreasing cost
     int *flags = nullptr;                  // one non-finite flag per problem
+    unsigned long long *snorm = nullptr;   // two norm bounds per problem (bit patterns)
+    int *pbits = nullptr;                  // the bits P of every problem in the current run
+    double limit = 0.0;                    // M / 2 of the plan's moduli, with margin
     int *expo = nullptr;                   // exponents E and F of every problem
     void *ws = nullptr;                    // residues
     hipStream_t capture = nullptr;
@@ -100,19 +104,26 @@ __device__ __forceinline__ double vec_at(const double *x, int64_t ld, bool conti
     return contig ? x[l + r * ld] : x[r + l * ld];
 }
 
-// Exponents of the rows of op(A) and the columns of op(B), and each problem's
-// non-finite flag (which gz_clear_flags zeroes first).  Workgroup (p, y):
-// problem p, operand y & 1 (0: op(A) rows, 1: op(B) columns), vectors
-// [64 (y >> 1), 64 (y >> 1) + 64); entries in [R, Rp) get exponent 0.  A
-// strided operand (vector r at x[r + l * ld]) is read by lane = vector, the
-// four waves taking every fourth l with eight loads in flight; a contiguous
-// one (x[l + r * ld]) by lanes along l, a wave taking 16 of the vectors four
-// at a time, and reduced across the lanes.
-__global__ void gz_clear_flags(int *flags, int n) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) flags[i] = 0;
+// Exponents of the rows of op(A) and the columns of op(B), each problem's
+// non-finite flag, and the norm bounds of its scaled vectors (gz_clear zeroes
+// the flags and bounds first).  Workgroup (p, y): problem p, operand y & 1
+// (0: op(A) rows, 1: op(B) columns), vectors [64 (y >> 1), 64 (y >> 1) + 64);
+// entries in [R, Rp) get exponent 0.  A strided operand (vector r at
+// x[r + l * ld]) is read by lane = vector, the four waves taking every fourth
+// l with eight loads in flight; a contiguous one (x[l + r * ld]) by lanes
+// along l, a wave taking 16 of the vectors four at a time, and reduced across
+// the lanes.  snorm[2p + op] receives max ||x_r||^2 2^-2E_r over the vectors
+// (as the bit pattern of a non-negative double; +inf when an exponent lies
+// outside [-NORM_EXP_LIMIT, NORM_EXP_LIMIT], as in the single-problem path).
+__global__ void gz_clear(int *flags, unsigned long long *snorm, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        flags[i] = 0;
+        snorm[2 * i] = 0ull;
+        snorm[2 * i + 1] = 0ull;
+    }
 }
 
-__global__ __launch_bounds__(256) void gz_exponents(const OzProblem *probs, int *flags) {
+__global__ __launch_bounds__(256) void gz_exponents(const OzProblem *probs, int *flags, unsigned long long *snorm) {
     const OzProblem &q = probs[blockIdx.x];
     if (q.scale_only) return;
     const bool opA = (blockIdx.y & 1) == 0;
@@ -123,36 +134,48 @@ __global__ __launch_bounds__(256) void gz_exponents(const OzProblem *probs, int 
     const int64_t ld = opA ? q.lda : q.ldb;
     const bool contig = opA ? q.ta != 0 : q.tb == 0;
     __shared__ unsigned long long part[4][64];
+    __shared__ double psq[4][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     auto bits = [](double v) { return (unsigned long long)__double_as_longlong(fabs(v)); };
     if (!contig) {
         const int64_t r = r0 + lane;
         unsigned long long best = 0;
+        double sq = 0.0;
         if (r < R) {
             int64_t l = wave;
             for (; l + 28 < K; l += 32) {
-                unsigned long long b[8];
+                double w[8];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) b[u] = bits(x[r + (l + 4 * u) * ld]);
+                for (int u = 0; u < 8; ++u) w[u] = x[r + (l + 4 * u) * ld];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) best = b[u] > best ? b[u] : best;
+                for (int u = 0; u < 8; ++u) {
+                    const unsigned long long b = bits(w[u]);
+                    best = b > best ? b : best;
+                    sq = fma(w[u], w[u], sq);
+                }
             }
             for (; l < K; l += 4) {
-                const unsigned long long b = bits(x[r + l * ld]);
+                const double w = x[r + l * ld];
+                const unsigned long long b = bits(w);
                 best = b > best ? b : best;
+                sq = fma(w, w, sq);
             }
         }
         part[wave][lane] = best;
+        psq[wave][lane] = sq;
     } else {
         for (int v0 = wave; v0 < 64; v0 += 16) {
             unsigned long long b[4] = {0, 0, 0, 0};
+            double q2[4] = {0.0, 0.0, 0.0, 0.0};
             for (int64_t l = lane; l < K; l += 64) {
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int64_t r = r0 + v0 + 4 * u;
                     if (r < R) {
-                        const unsigned long long c = bits(x[l + r * ld]);
+                        const double w = x[l + r * ld];
+                        const unsigned long long c = bits(w);
                         b[u] = c > b[u] ? c : b[u];
+                        q2[u] = fma(w, w, q2[u]);
                     }
                 }
             }
@@ -162,14 +185,20 @@ __global__ __launch_bounds__(256) void gz_exponents(const OzProblem *probs, int 
                 for (int u = 0; u < 4; ++u) {
                     const unsigned long long o = __shfl_xor(b[u], off);
                     b[u] = o > b[u] ? o : b[u];
+                    q2[u] += __shfl_xor(q2[u], off);
                 }
             }
             if (lane < 4) {
                 unsigned long long mine = b[0];
+                double qs = q2[0];
 #pragma unroll
-                for (int u = 1; u < 4; ++u) mine = lane == u ? b[u] : mine;
+                for (int u = 1; u < 4; ++u) {
+                    mine = lane == u ? b[u] : mine;
+                    qs = lane == u ? q2[u] : qs;
+                }
                 const int v = v0 + 4 * lane;
                 part[v & 3][v >> 2] = mine;
+                psq[v & 3][v >> 2] = qs;
             }
         }
     }
@@ -177,19 +206,52 @@ __global__ __launch_bounds__(256) void gz_exponents(const OzProblem *probs, int 
     if (threadIdx.x < 64) {
         const int v = threadIdx.x;
         const int64_t r = r0 + v;
+        double nrm = 0.0;   // ||x_r||^2 2^-2E_r of this vector
         if (r < Rp) {
             unsigned long long b;
+            double sq;
             if (!contig) {
                 b = part[0][v];
+                sq = psq[0][v];
 #pragma unroll
-                for (int w = 1; w < 4; ++w) b = part[w][v] > b ? part[w][v] : b;
+                for (int w = 1; w < 4; ++w) {
+                    b = part[w][v] > b ? part[w][v] : b;
+                    sq += psq[w][v];
+                }
             } else {
                 b = part[v & 3][v >> 2];
+                sq = psq[v & 3][v >> 2];
             }
             if (r < R && b >= 0x7ff0000000000000ull) atomicOr(&flags[blockIdx.x], 1);   // inf or NaN
-            (opA ? q.E : q.F)[r] = r < R ? row_exponent(b, 0) : 0;
+            const int e = r < R ? row_exponent(b, 0) : 0;
+            (opA ? q.E : q.F)[r] = e;
+            if (r < R && e != ZERO_EXPONENT)
+                nrm = (e < -NORM_EXP_LIMIT || e > NORM_EXP_LIMIT) ? INFINITY : ldexp(sq, -2 * e);
+            if (nrm != nrm) nrm = INFINITY;
         }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) nrm = fmax(nrm, __shfl_xor(nrm, off));
+        if (v == 0) atomicMax(&snorm[2 * blockIdx.x + (opA ? 0 : 1)], (unsigned long long)__double_as_longlong(nrm));
     }
+}
+
+// The bits P of every problem for this run: the entrywise bound's P (set at
+// plan creation for the plan's moduli count) raised as far as the norm bound
+// of the operands now present allows (norm_bits; the same rule and
+// arithmetic as the single-problem path, whose results this keeps
+// bit-identical).
+__global__ void gz_bits(const OzProblem *probs, const int *flags, const unsigned long long *snorm, int *pbits,
+                        int nprob, double limit) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nprob) return;
+    const OzProblem &q = probs[i];
+    int P = q.P;
+    if (!q.scale_only && !flags[i]) {
+        const double sa = sqrt(__longlong_as_double((long long)snorm[2 * i])) * (1.0 + 1e-9);
+        const double sb = sqrt(__longlong_as_double((long long)snorm[2 * i + 1])) * (1.0 + 1e-9);
+        P = norm_bits(limit, 0.5 * sqrt((double)q.k), sa, sb, q.P);
+    }
+    pbits[i] = P;
 }
 
 // Plan creation only: the error factor gamma_p of each problem (ozaki.hip's
@@ -262,9 +324,10 @@ __global__ __launch_bounds__(256) void gz_gamma(const OzProblem *probs, const in
 // along the vector, so the residue stores are contiguous runs.  Strided
 // operand: 64 x 64 (vector x k) tiles are read along the vectors into LDS and
 // transposed, four threads per vector writing 64 contiguous bytes per plane.
-__global__ __launch_bounds__(256) void gz_split(const OzProblem *probs, GzModuli md) {
+__global__ __launch_bounds__(256) void gz_split(const OzProblem *probs, const int *pbits, GzModuli md) {
     const OzProblem &q = probs[blockIdx.x];
     if (q.scale_only) return;
+    const int P = pbits[blockIdx.x];
     const bool opA = (blockIdx.y & 1) == 0;
     const int64_t r0 = (int64_t)(blockIdx.y >> 1) * 64;
     const int64_t R = opA ? q.m : q.n, Rp = opA ? q.Mp : q.Np;
@@ -290,7 +353,7 @@ __global__ __launch_bounds__(256) void gz_split(const OzProblem *probs, GzModuli
             double v[16];
 #pragma unroll
             for (int j = 0; j < 16; ++j) v[j] = k0 + j < q.k ? x[k0 + j + r * ld] : 0.0;
-            moduli_residues16(v, ex[r], q.P, md.count, md.m, md.inv_m, dst + r * q.Kp + k0, plane);
+            moduli_residues16(v, ex[r], P, md.count, md.m, md.inv_m, dst + r * q.Kp + k0, plane);
         }
         return;
     }
@@ -315,7 +378,7 @@ __global__ __launch_bounds__(256) void gz_split(const OzProblem *probs, GzModuli
         double v[16];
 #pragma unroll
         for (int j = 0; j < 16; ++j) v[j] = tile[16 * kc + j][row];
-        moduli_residues16(v, r < R ? e : 0, q.P, md.count, md.m, md.inv_m, dst + r * q.Kp + k0 + 16 * kc, plane);
+        moduli_residues16(v, r < R ? e : 0, P, md.count, md.m, md.inv_m, dst + r * q.Kp + k0 + 16 * kc, plane);
     }
 }
 
@@ -419,8 +482,10 @@ __global__ __launch_bounds__(256) void gz_products(const OzProblem *probs, int n
 // takes 4 consecutive rows of one column.  Problems with k == 0 or alpha == 0
 // only scale C; problems flagged non-finite are computed by FP64 dot products.
 template <int NL>
-__global__ __launch_bounds__(256) void gz_recombine(const OzProblem *probs, const int *flags, CrtConst cc) {
+__global__ __launch_bounds__(256) void gz_recombine(const OzProblem *probs, const int *flags, const int *pbits,
+                                                    CrtConst cc) {
     const OzProblem &q = probs[blockIdx.x];
+    const int P = pbits[blockIdx.x];
     const int64_t quads = (q.m + 3) / 4, items = quads * q.n;
     const bool plain = flags[blockIdx.x] != 0;
     for (int64_t it = threadIdx.x + (int64_t)blockIdx.y * blockDim.x; it < items;
@@ -456,7 +521,7 @@ __global__ __launch_bounds__(256) void gz_recombine(const OzProblem *probs, cons
             const int fj = q.F[j];
 #pragma unroll
             for (int e = 0; e < 4; ++e)
-                if (i0 + e < q.m) val[e] = q.alpha * ldexp(crt_value<NL>(s[e], cc), q.E[i0 + e] + fj - 2 * q.P);
+                if (i0 + e < q.m) val[e] = q.alpha * ldexp(crt_value<NL>(s[e], cc), q.E[i0 + e] + fj - 2 * P);
         }
         for (int e = 0; e < 4 && i0 + e < q.m; ++e) {
             double *cp = q.c + i0 + e + j * q.ldc;
@@ -494,6 +559,8 @@ int grouped_ozaki_destroy(cosma_amd_grouped_ozaki_t p) {
     if (p->capture) (void)hipStreamDestroy(p->capture);
     if (p->dev) (void)hipFree(p->dev);
     if (p->flags) (void)hipFree(p->flags);
+    if (p->snorm) (void)hipFree(p->snorm);
+    if (p->pbits) (void)hipFree(p->pbits);
     if (p->expo) (void)hipFree(p->expo);
     if (p->ws) (void)hipFree(p->ws);
     delete p;
@@ -504,17 +571,19 @@ int grouped_ozaki_run(cosma_amd_grouped_ozaki_t p, hipStream_t s) {
     if (p->nprob == 0) return COSMA_AMD_OK;
     CA_HIP(hipSetDevice(p->h->device));
     const dim3 gv((unsigned)p->nprob, (unsigned)(2 * p->vgroups));
-    hipLaunchKernelGGL(gz_clear_flags, dim3((unsigned)((p->nprob + 255) / 256)), dim3(256), 0, s, p->flags, p->nprob);
-    hipLaunchKernelGGL(gz_exponents, gv, dim3(256), 0, s, p->dev, p->flags);
-    hipLaunchKernelGGL(gz_split, gv, dim3(256), 0, s, p->dev, p->md);
+    const dim3 g1((unsigned)((p->nprob + 255) / 256));
+    hipLaunchKernelGGL(gz_clear, g1, dim3(256), 0, s, p->flags, p->snorm, p->nprob);
+    hipLaunchKernelGGL(gz_exponents, gv, dim3(256), 0, s, p->dev, p->flags, p->snorm);
+    hipLaunchKernelGGL(gz_bits, g1, dim3(256), 0, s, p->dev, p->flags, p->snorm, p->pbits, p->nprob, p->limit);
+    hipLaunchKernelGGL(gz_split, gv, dim3(256), 0, s, p->dev, p->pbits, p->md);
     if (p->total_tiles > 0)
         hipLaunchKernelGGL(gz_products, dim3((unsigned)p->grid), dim3(256), 0, s, p->dev, p->nprob, p->total_tiles,
                            p->md);
     const dim3 gr((unsigned)p->nprob, GZ_RECOMBINE_Y);
     if (crt_limbs(p->moduli) == 3)
-        hipLaunchKernelGGL(gz_recombine<3>, gr, dim3(256), 0, s, p->dev, p->flags, p->cc);
+        hipLaunchKernelGGL(gz_recombine<3>, gr, dim3(256), 0, s, p->dev, p->flags, p->pbits, p->cc);
     else
-        hipLaunchKernelGGL(gz_recombine<NLIMB>, gr, dim3(256), 0, s, p->dev, p->flags, p->cc);
+        hipLaunchKernelGGL(gz_recombine<NLIMB>, gr, dim3(256), 0, s, p->dev, p->flags, p->pbits, p->cc);
     CA_HIP(hipGetLastError());
     return COSMA_AMD_OK;
 }
@@ -679,6 +748,9 @@ int grouped_ozaki_create(cosma_amd_handle_t h, const cosma_amd_gemm_desc *d, int
     const size_t ndesc = sorted.size() * sizeof(OzProblem), nstart = sorted.size() * sizeof(int64_t);
     if ((rc = alloc(reinterpret_cast<void **>(&p->dev), ndesc + nstart))) return fail(rc);
     if ((rc = alloc(reinterpret_cast<void **>(&p->flags), sorted.size() * sizeof(int)))) return fail(rc);
+    if ((rc = alloc(reinterpret_cast<void **>(&p->snorm), 2 * sorted.size() * sizeof(unsigned long long))))
+        return fail(rc);
+    if ((rc = alloc(reinterpret_cast<void **>(&p->pbits), sorted.size() * sizeof(int)))) return fail(rc);
 
     // exponents first (their space does not depend on the moduli count)
     int64_t nexp = 0;
@@ -705,29 +777,39 @@ int grouped_ozaki_create(cosma_amd_handle_t h, const cosma_amd_gemm_desc *d, int
     // none is given, and give the plan's estimate either way
     if ((rc = upload())) return fail(rc);
     std::vector<double> gamma(sorted.size());
+    std::vector<unsigned long long> hnorm(2 * sorted.size());
     {
         double *dgamma = nullptr;
         if ((rc = alloc(reinterpret_cast<void **>(&dgamma), sorted.size() * sizeof(double)))) return fail(rc);
         const dim3 gv((unsigned)p->nprob, (unsigned)(2 * p->vgroups));
-        hipLaunchKernelGGL(gz_clear_flags, dim3((unsigned)((p->nprob + 255) / 256)), dim3(256), 0, 0, p->flags,
+        hipLaunchKernelGGL(gz_clear, dim3((unsigned)((p->nprob + 255) / 256)), dim3(256), 0, 0, p->flags, p->snorm,
                            p->nprob);
-        hipLaunchKernelGGL(gz_exponents, gv, dim3(256), 0, 0, p->dev, p->flags);
+        hipLaunchKernelGGL(gz_exponents, gv, dim3(256), 0, 0, p->dev, p->flags, p->snorm);
         hipLaunchKernelGGL(gz_gamma, dim3((unsigned)p->nprob), dim3(256), 0, 0, p->dev, p->flags, dgamma);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) e = hipMemcpy(gamma.data(), dgamma, gamma.size() * sizeof(double), hipMemcpyDeviceToHost);
+        if (e == hipSuccess)
+            e = hipMemcpy(hnorm.data(), p->snorm, hnorm.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost);
         (void)hipFree(dgamma);
         if (e != hipSuccess) {
             set_error("grouped ozaki: error estimation failed (%s)", hipGetErrorString(e));
             return fail(COSMA_AMD_ERR_HIP);
         }
     }
-    // the doubled estimate of the worst problem for n moduli
+    // the doubled estimate of the worst problem for n moduli, with the bits
+    // each problem gets in a run on the operands present now (gz_bits' rule)
     auto worst = [&](int n) {
         double w = 0.0;
+        const double limit = moduli_half_limit(n);
         for (size_t i = 0; i < sorted.size(); ++i) {
             if (sorted[i].scale_only || gamma[i] == 0.0) continue;
-            const int P = std::min(moduli_bits(n, sorted[i].k), MAX_P);
+            int P = std::min(moduli_bits(n, sorted[i].k), MAX_P);
             if (P < 1) return (double)INFINITY;
+            double na, nb;
+            std::memcpy(&na, &hnorm[2 * i], sizeof(double));
+            std::memcpy(&nb, &hnorm[2 * i + 1], sizeof(double));
+            P = norm_bits(limit, 0.5 * std::sqrt((double)sorted[i].k), std::sqrt(na) * (1.0 + 1e-9),
+                          std::sqrt(nb) * (1.0 + 1e-9), P);
             w = std::max(w, 2.0 * gamma[i] * std::ldexp(1.0, -P));
         }
         return w;
@@ -738,6 +820,7 @@ int grouped_ozaki_create(cosma_amd_handle_t h, const cosma_amd_gemm_desc *d, int
     }
     p->estimate = worst(moduli);
     p->moduli = moduli;
+    p->limit = moduli_half_limit(moduli);
     for (auto &g : sorted) {
         if (g.scale_only) continue;
         g.P = std::min(moduli_bits(moduli, g.k), MAX_P);

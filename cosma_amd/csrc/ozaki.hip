@@ -18,6 +18,9 @@
 //     recovers the exact integer A'B' (|A'B'| < M / 2, M = prod m_l), which is
 //     scaled back by 2^(E_i + F_j - 2P).  The only approximation is the
 //     rounding of the inputs to P bits (relative to their row / column max).
+//     P is the largest value for which either the entrywise bound
+//     |A'B'| <= k 2^(2P) or the row / column 2-norm bound (norm_bits) keeps
+//     |A'B'| below M / 2.
 //   * slices: the scaled entries are written as sums of s signed 7-bit digits
 //     (base 2^7, round to nearest, the leading digit within [-127, 127]);
 //     the slice products with t + u <= s + 1 are accumulated exactly in INT32
@@ -48,62 +51,91 @@ __global__ void clear_u64(unsigned long long *p, int64_t n) {
 
 // Statistics of op(X) seen as R vectors of length K (the rows of op(A), the
 // columns of op(B)): the max |x| of every vector (as bit patterns, for the
-// exponents) and, for every k, the sum of squares across the vectors (for the
-// error estimate).  No LDS: a wave reads 64 consecutive elements along the
+// exponents), the sum of squares of every vector (for the bound on |A'B'|)
+// and, for every k, the sum of squares across the vectors (for the error
+// estimate).  No LDS: a wave reads 64 consecutive elements along the
 // contiguous dimension per load, keeps the reduction along the other
 // dimension in registers, and reduces across its lanes with shuffles.
 // CONTIG (element (r, k) at x[k + r * ld]): lane = k, wave = 64 vectors, one
-// wave max per vector; otherwise (x[r + k * ld]): lane = r, wave = 64 k, one
-// wave sum per k.  Workgroups cover 64 lanes x STATS_SPAN of the other
-// dimension (4 waves x STATS_SPAN / 4).
+// wave max and sum per vector and STATS_KC * 64 k; otherwise (x[r + k * ld]):
+// lane = r, wave = 64 k, one wave sum per k.  Workgroups cover STATS_KC * 64
+// (CONTIG) or 64 lanes x STATS_SPAN of the other dimension (4 waves x
+// STATS_SPAN / 4).
 constexpr int STATS_SPAN = 256;
+constexpr int STATS_KC = 8;   // 64-wide k chunks per workgroup (contiguous vectors)
 
 template <bool CONTIG>
 __global__ __launch_bounds__(256) void stats_kernel(const double *__restrict__ x, int64_t ld, int64_t R, int64_t K,
-                                                    unsigned long long *maxbits, double *ksq) {
+                                                    unsigned long long *maxbits, double *ksq, double *rsq) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     constexpr int PER_WAVE = STATS_SPAN / 4, U = 8;
     if (CONTIG) {
-        const int64_t k = (int64_t)blockIdx.x * 64 + lane;
+        // A workgroup covers STATS_KC chunks of 64 consecutive k; each lane
+        // accumulates its k of every chunk for a group of U vectors before
+        // the group's cross-lane reductions, so there is one reduction per
+        // STATS_KC * 64 elements of a vector (and 8 x 8 loads in flight).
+        const int64_t kbase = (int64_t)blockIdx.x * 64 * STATS_KC + lane;
         const int64_t rb = (int64_t)blockIdx.y * STATS_SPAN + wave * PER_WAVE;
-        const bool kin = k < K;
-        double ss = 0.0;
+        double ss[STATS_KC];
+#pragma unroll
+        for (int c = 0; c < STATS_KC; ++c) ss[c] = 0.0;
         for (int q0 = 0; q0 < PER_WAVE; q0 += U) {
-            double v[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int64_t r = rb + q0 + u;
-                v[u] = (kin && r < R) ? x[k + r * ld] : 0.0;
-            }
-            // U independent lane reductions, stepped together so their
-            // shuffle latencies overlap
             unsigned long long b[U];
+            double q[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                ss = fma(v[u], v[u], ss);
-                b[u] = __double_as_longlong(fabs(v[u]));
+                b[u] = 0;
+                q[u] = 0.0;
             }
+#pragma unroll
+            for (int c = 0; c < STATS_KC; ++c) {
+                const int64_t k = kbase + 64 * c;
+                double v[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int64_t r = rb + q0 + u;
+                    v[u] = (k < K && r < R) ? x[k + r * ld] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    ss[c] = fma(v[u], v[u], ss[c]);
+                    q[u] = fma(v[u], v[u], q[u]);
+                    const unsigned long long bb = __double_as_longlong(fabs(v[u]));
+                    b[u] = bb > b[u] ? bb : b[u];
+                }
+            }
+            // U independent lane reductions (max and sum of squares), stepped
+            // together so their shuffle latencies overlap
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) {
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     const unsigned long long o = __shfl_xor(b[u], off);
                     b[u] = o > b[u] ? o : b[u];
+                    q[u] += __shfl_xor(q[u], off);
                 }
             }
             if (lane < U && rb + q0 + lane < R) {
                 unsigned long long mine = b[0];
+                double qs = q[0];
 #pragma unroll
-                for (int u = 1; u < U; ++u) mine = lane == u ? b[u] : mine;
+                for (int u = 1; u < U; ++u) {
+                    mine = lane == u ? b[u] : mine;
+                    qs = lane == u ? q[u] : qs;
+                }
                 atomicMax(&maxbits[rb + q0 + lane], mine);
+                atomicAdd(&rsq[rb + q0 + lane], qs);
             }
         }
-        if (kin) atomicAdd(&ksq[k], ss);
+#pragma unroll
+        for (int c = 0; c < STATS_KC; ++c)
+            if (kbase + 64 * c < K) atomicAdd(&ksq[kbase + 64 * c], ss[c]);
     } else {
         const int64_t r = (int64_t)blockIdx.x * 64 + lane;
         const int64_t kb = (int64_t)blockIdx.y * STATS_SPAN + wave * PER_WAVE;
         const bool rin = r < R;
         unsigned long long best = 0;
+        double rs = 0.0;
         for (int q0 = 0; q0 < PER_WAVE; q0 += U) {
             double v[U];
 #pragma unroll
@@ -117,6 +149,7 @@ __global__ __launch_bounds__(256) void stats_kernel(const double *__restrict__ x
                 const unsigned long long b = __double_as_longlong(fabs(v[u]));
                 best = b > best ? b : best;
                 sq[u] = v[u] * v[u];
+                rs += sq[u];
             }
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) {
@@ -130,7 +163,10 @@ __global__ __launch_bounds__(256) void stats_kernel(const double *__restrict__ x
                 atomicAdd(&ksq[kb + q0 + lane], mine);
             }
         }
-        if (rin) atomicMax(&maxbits[r], best);
+        if (rin) {
+            atomicMax(&maxbits[r], best);
+            atomicAdd(&rsq[r], rs);
+        }
     }
 }
 
@@ -319,6 +355,8 @@ int moduli_bits(int n, int64_t k) {
     return (int)std::floor((lm - 1.0 - std::log2((double)std::max<int64_t>(k, 1)) - 1e-6) / 2.0);
 }
 
+double moduli_half_limit(int n) { return std::exp2(log2_moduli(n) - 1.0 - 1e-6); }
+
 namespace {
 // Little-endian 256-bit unsigned integer, enough for M = prod of up to 20
 // moduli (< 2^160) and the CRT weights W_l < M.
@@ -451,40 +489,45 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
     };
     int rc;
 
-    // ---- 1. row / column maxima, per-k sums of squares, exponents
-    // aux: maxA [Mp], maxB [Np], ksqA [Kp], ksqB [Kp], flag [1] (cleared
-    // together, 8 bytes each), then the exponents E [Mp] and F [Np]
-    const int64_t nclear = Mp + Np + 2 * Kp + 1;
+    // ---- 1. row / column maxima and sums of squares, per-k sums of squares,
+    // exponents.  aux: maxA [Mp], maxB [Np], ksqA [Kp], ksqB [Kp], rsqA [Mp],
+    // rsqB [Np], flag [1] (cleared together, 8 bytes each), then the
+    // exponents E [Mp] and F [Np]
+    const int64_t nclear = 2 * (Mp + Np) + 2 * Kp + 1;
     if ((rc = grow(h->ozaki_aux, al((size_t)nclear * 8 + (size_t)(Mp + Np) * 4)))) return rc;
     unsigned long long *maxA = static_cast<unsigned long long *>(h->ozaki_aux.ptr);
     unsigned long long *maxB = maxA + Mp;
     double *ksqA = reinterpret_cast<double *>(maxB + Np);
     double *ksqB = ksqA + Kp;
-    int *flag = reinterpret_cast<int *>(ksqB + Kp);
-    int *E = reinterpret_cast<int *>(ksqB + Kp + 1);
+    double *rsqA = ksqB + Kp;
+    double *rsqB = rsqA + Mp;
+    int *flag = reinterpret_cast<int *>(rsqB + Np);
+    int *E = reinterpret_cast<int *>(rsqB + Np + 1);
     int *F = E + Mp;
     hipLaunchKernelGGL(clear_u64, dim3(grid_for(nclear)), dim3(256), 0, s, maxA, nclear);
     const bool a_rows_contig = ta == 'T', b_rows_contig = tb == 'N';   // op rows contiguous in k
     // grid: x over the contiguous dimension in 64s, y over the other in STATS_SPAN
     auto stats_grid = [](int64_t R, int64_t K, bool contig) {
-        const int64_t fast = contig ? K : R, slow = contig ? R : K;
-        return dim3((unsigned)((fast + 63) / 64), (unsigned)((slow + STATS_SPAN - 1) / STATS_SPAN));
+        const int64_t fast = contig ? K : R, slow = contig ? R : K, span = contig ? 64 * STATS_KC : 64;
+        return dim3((unsigned)((fast + span - 1) / span), (unsigned)((slow + STATS_SPAN - 1) / STATS_SPAN));
     };
-    if (a_rows_contig) hipLaunchKernelGGL(stats_kernel<true>, stats_grid(m, k, true), dim3(256), 0, s, a, lda, m, k, maxA, ksqA);
-    else hipLaunchKernelGGL(stats_kernel<false>, stats_grid(m, k, false), dim3(256), 0, s, a, lda, m, k, maxA, ksqA);
-    if (b_rows_contig) hipLaunchKernelGGL(stats_kernel<true>, stats_grid(n, k, true), dim3(256), 0, s, b, ldb, n, k, maxB, ksqB);
-    else hipLaunchKernelGGL(stats_kernel<false>, stats_grid(n, k, false), dim3(256), 0, s, b, ldb, n, k, maxB, ksqB);
+    if (a_rows_contig) hipLaunchKernelGGL(stats_kernel<true>, stats_grid(m, k, true), dim3(256), 0, s, a, lda, m, k, maxA, ksqA, rsqA);
+    else hipLaunchKernelGGL(stats_kernel<false>, stats_grid(m, k, false), dim3(256), 0, s, a, lda, m, k, maxA, ksqA, rsqA);
+    if (b_rows_contig) hipLaunchKernelGGL(stats_kernel<true>, stats_grid(n, k, true), dim3(256), 0, s, b, ldb, n, k, maxB, ksqB, rsqB);
+    else hipLaunchKernelGGL(stats_kernel<false>, stats_grid(n, k, false), dim3(256), 0, s, b, ldb, n, k, maxB, ksqB, rsqB);
     hipLaunchKernelGGL(exponents_kernel, dim3(grid_for(m)), dim3(256), 0, s, maxA, m, slices, E, flag);
     hipLaunchKernelGGL(exponents_kernel, dim3(grid_for(n)), dim3(256), 0, s, maxB, n, slices, F, flag);
     CA_HIP(hipGetLastError());
     int nonfinite = 0;
     std::vector<int> hE(m), hF(n);
-    std::vector<double> hkA(k), hkB(k);
+    std::vector<double> hkA(k), hkB(k), hrA(m), hrB(n);
     CA_HIP(hipMemcpyAsync(&nonfinite, flag, sizeof(int), hipMemcpyDeviceToHost, s));
     CA_HIP(hipMemcpyAsync(hE.data(), E, m * sizeof(int), hipMemcpyDeviceToHost, s));
     CA_HIP(hipMemcpyAsync(hF.data(), F, n * sizeof(int), hipMemcpyDeviceToHost, s));
     CA_HIP(hipMemcpyAsync(hkA.data(), ksqA, k * sizeof(double), hipMemcpyDeviceToHost, s));
     CA_HIP(hipMemcpyAsync(hkB.data(), ksqB, k * sizeof(double), hipMemcpyDeviceToHost, s));
+    CA_HIP(hipMemcpyAsync(hrA.data(), rsqA, m * sizeof(double), hipMemcpyDeviceToHost, s));
+    CA_HIP(hipMemcpyAsync(hrB.data(), rsqB, n * sizeof(double), hipMemcpyDeviceToHost, s));
     CA_HIP(hipStreamSynchronize(s));
     if (nonfinite) {
         // Inf / NaN cannot be split; the FP64 MFMA DGEMM propagates them as IEEE does.
@@ -503,6 +546,28 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
     // meets the tolerance.
     double gamma = error_gamma(hE, hF, hkA, hkB);
     if (!std::isfinite(gamma)) gamma = std::ldexp(1.0, 30);    // no estimate: most precise count
+    // Norm bounds of the scaled rows / columns for norm_bits: max over the
+    // vectors of ||x|| 2^-E, slightly inflated for the rounding of the FP64
+    // sums.  Only used when every exponent lies in [-480, 480], where the
+    // squares of a vector's largest entries neither overflow nor underflow
+    // (entries that do underflow are negligible beside them); otherwise P
+    // comes from the entrywise bound alone.
+    auto norm_max = [&](const std::vector<int> &ex, const std::vector<double> &rs) {
+        double mx = 0.0;
+        for (size_t i = 0; i < ex.size(); ++i) {
+            if (ex[i] == ZERO_EXPONENT) continue;
+            const double v = (ex[i] < -NORM_EXP_LIMIT || ex[i] > NORM_EXP_LIMIT) ? INFINITY
+                                                                                  : std::ldexp(rs[i], -2 * ex[i]);
+            mx = v > mx ? v : (v != v ? INFINITY : mx);
+        }
+        return std::sqrt(mx) * (1.0 + 1e-9);
+    };
+    const double sa = norm_max(hE, hrA), sb = norm_max(hF, hrB);
+    const double hk = 0.5 * std::sqrt((double)k);
+    auto bits = [&](int cnt) {
+        const int pe = std::min(moduli_bits(cnt, k), MAX_P);
+        return norm_bits(moduli_half_limit(cnt), hk, sa, sb, pe);
+    };
     int P = 0;
     // Slices: the s-digit remainders and the dropped products of weight
     // 2^-7(s+2) and beyond give about gamma * 2 s 2^-7s; measured errors
@@ -510,14 +575,14 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
     // estimate for slices is doubled twice.
     auto est = [&](int cnt) {
         return slices ? 4.0 * gamma * 2.0 * cnt * std::ldexp(1.0, -7 * cnt)
-                      : 2.0 * gamma * std::ldexp(1.0, -std::min(moduli_bits(cnt, k), MAX_P));
+                      : 2.0 * gamma * std::ldexp(1.0, -bits(cnt));
     };
     if (!slices) {
         if (count <= 0) {
             count = 2;
-            while (count < MAX_MODULI && !(moduli_bits(count, k) >= 1 && est(count) <= tol)) ++count;
+            while (count < MAX_MODULI && !(bits(count) >= 1 && est(count) <= tol)) ++count;
         }
-        P = std::min(moduli_bits(count, k), MAX_P);
+        P = bits(count);
         if (P < 1) {
             set_error("ozaki: %d moduli leave no precision for k = %lld", count, (long long)k);
             return COSMA_AMD_ERR_UNSUPPORTED;

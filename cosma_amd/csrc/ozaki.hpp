@@ -35,6 +35,28 @@ struct CrtConst {
 double log2_moduli(int n);
 // Largest P with k * 2^(2P) < M / 2 for the first n moduli.
 int moduli_bits(int n, int64_t k);
+// M / 2 for the first n moduli, reduced by the same relative margin as
+// moduli_bits: the bound the integer product must stay below.
+double moduli_half_limit(int n);
+
+// Largest P in (pmin, MAX_P] with (2^P sa + h)(2^P sb + h) < limit, else
+// pmin.  With sa >= ||a_i|| 2^-E_i for every row of op(A), sb likewise for
+// the columns of op(B), h = sqrt(k) / 2 and limit = moduli_half_limit(n),
+// the integer inputs a' = rint(a 2^(P - E)) satisfy ||a'_i|| <= 2^P sa + h
+// (each entry moves by at most 1/2), so by Cauchy-Schwarz |A'B'| stays
+// inside (-M/2, M/2).  Plain IEEE arithmetic only, so the host and the
+// device agree on the result.
+__host__ __device__ inline int norm_bits(double limit, double h, double sa, double sb, int pmin) {
+    for (int P = MAX_P; P > pmin; --P) {
+        const double lhs = (ldexp(sa, P) + h) * (ldexp(sb, P) + h);
+        if (lhs < limit) return P;   // false for NaN / inf
+    }
+    return pmin;
+}
+
+// Exponent range in which the squares of a vector's largest entries neither
+// overflow nor underflow in FP64, so its sum of squares bounds its norm.
+constexpr int NORM_EXP_LIMIT = 480;
 void crt_constants(int n, CrtConst &cc);
 // CRT limbs needed for n moduli: 3 while M < 2^122, else 4.
 inline int crt_limbs(int n) { return log2_moduli(n) < 3 * LIMB - 1 ? 3 : NLIMB; }

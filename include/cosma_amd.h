@@ -89,7 +89,9 @@ enum {
 typedef struct {
     int method;             /* method used; -1 if A or B held Inf/NaN (FP64 MFMA DGEMM used instead) */
     int count;              /* moduli or slices used */
-    int bits;               /* bits kept per input entry relative to its row/column maximum */
+    int bits;               /* bits kept per input entry relative to its row/column maximum (moduli: the
+                               largest P for which the entrywise or the row/column-norm bound keeps |A'B'|
+                               below half the moduli product) */
     int64_t int8_products;  /* INT8 (BF16 for COSMA_AMD_OZAKI_MODULI_BF16) GEMMs of size m x n x k computed */
     double error_bound;     /* estimated relative Frobenius error (doubled), from the inputs' row statistics */
 } cosma_amd_ozaki_info;

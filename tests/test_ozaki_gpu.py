@@ -122,6 +122,30 @@ def test_moduli_equals_exact_product_of_rounded_inputs(handle, count, ta, tb):
     assert (diff <= 2 * np.spacing(np.abs(ref))).all(), diff.max()
 
 
+@pytest.mark.parametrize("method", [0, 2])
+def test_norm_bound_raises_bits_and_stays_exact(handle, method):
+    """Rows of op(A) / columns of op(B) with one entry at the maximum and the
+    rest far below it: their 2-norms bound |A'B'| much more tightly than
+    k 2^(2P), so the bits P chosen for 8 moduli exceed the entrywise limit
+    floor((log2 M - 1 - log2 k) / 2), and the result must still be the exact
+    product of the P-bit-rounded inputs (|A'B'| < M / 2 is what makes the
+    residues recombine exactly)."""
+    m, n, k, count = 48, 40, 512, 8
+    rng = np.random.default_rng(3)
+    X = rng.uniform(-1, 1, (m, k)) * 2.0 ** -12
+    Y = rng.uniform(-1, 1, (k, n)) * 2.0 ** -12
+    X[np.arange(m), rng.integers(0, k, m)] = 1.0
+    Y[rng.integers(0, k, n), np.arange(n)] = -1.0
+    X, Y = np.asfortranarray(X), np.asfortranarray(Y)
+    got, info = run_ozaki(handle, "N", "N", m, n, k, 1.0, X, Y, 0.0, np.zeros((m, n), order="F"), method,
+                          count=count)
+    log2M = sum(math.log2(x) for x in MODULI[:count])
+    entrywise = math.floor((log2M - 1 - math.log2(k) - 1e-6) / 2)
+    assert info["bits"] > entrywise, (info, entrywise)
+    ref = emulate_moduli(X, Y, info["bits"], 1.0)
+    assert (np.abs(got - ref) <= 2 * np.spacing(np.abs(ref))).all()
+
+
 @pytest.mark.parametrize("s", [1, 2, 4, 7, 8])
 def test_slices_match_emulation_bit_for_bit(handle, s):
     m, n, k = 41, 33, 200
