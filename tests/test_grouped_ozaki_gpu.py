@@ -131,6 +131,57 @@ def test_default_tolerance_against_oracle(handle):
     plan.close()
 
 
+def test_norm_bound_bits_match_single_calls(handle):
+    """Operands with one entry per row / column at the maximum and the rest
+    far below: the plan's device-side bits rule raises P well above the
+    entrywise limit, exactly as the single-problem path does, so the results
+    stay bit-identical to single calls (and exact products of the rounded
+    inputs), also after the operands change between graph replays."""
+    import torch
+    import cosma_amd as ca
+    rng = np.random.default_rng(13)
+    probs = make_problems(12, 17, max_dim=160)
+    for p in probs:
+        if p["k"] == 0:
+            continue
+        A = p["A"] * 2.0 ** -14
+        B = p["B"] * 2.0 ** -14
+        rows, cols = A.shape
+        if p["ta"] == "N":
+            A[np.arange(rows), rng.integers(0, cols, rows)] = 1.0
+        else:
+            A[rng.integers(0, rows, cols), np.arange(cols)] = 1.0
+        rows, cols = B.shape
+        if p["tb"] == "N":
+            B[rng.integers(0, rows, cols), np.arange(cols)] = -1.0
+        else:
+            B[np.arange(rows), rng.integers(0, cols, rows)] = -1.0
+        p["A"], p["B"] = np.asfortranarray(A), np.asfortranarray(B)
+        p["dA"].copy_(to_dev(p["A"], p["lda"])[0])
+        p["dB"].copy_(to_dev(p["B"], p["ldb"])[0])
+    reset_c(probs)
+    torch.cuda.synchronize()
+    expected = single_results(handle, probs, 8)
+    plan = ca.GroupedOzaki(handle, descs(probs), moduli=8)
+    plan.graph_run()
+    torch.cuda.synchronize()
+    assert same(results(probs), expected)
+    # new operands of the same shapes (uniform, no spikes): the replay
+    # recomputes the bits from them
+    for i, p in enumerate(probs):
+        p["A"] = np.asfortranarray(oracle.uniform(*p["A"].shape, 500 + i))
+        p["B"] = np.asfortranarray(oracle.uniform(*p["B"].shape, 600 + i))
+        p["dA"].copy_(to_dev(p["A"], p["lda"])[0])
+        p["dB"].copy_(to_dev(p["B"], p["ldb"])[0])
+    reset_c(probs)
+    torch.cuda.synchronize()
+    expected = single_results(handle, probs, 8)
+    plan.graph_run()
+    torch.cuda.synchronize()
+    assert same(results(probs), expected)
+    plan.close()
+
+
 def test_graph_replays_match_eager(handle, problems):
     import torch
     import cosma_amd as ca
