@@ -32,6 +32,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "ozaki.hpp"
@@ -518,17 +519,20 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
     hipLaunchKernelGGL(exponents_kernel, dim3(grid_for(m)), dim3(256), 0, s, maxA, m, slices, E, flag);
     hipLaunchKernelGGL(exponents_kernel, dim3(grid_for(n)), dim3(256), 0, s, maxB, n, slices, F, flag);
     CA_HIP(hipGetLastError());
-    int nonfinite = 0;
-    std::vector<int> hE(m), hF(n);
-    std::vector<double> hkA(k), hkB(k), hrA(m), hrB(n);
-    CA_HIP(hipMemcpyAsync(&nonfinite, flag, sizeof(int), hipMemcpyDeviceToHost, s));
-    CA_HIP(hipMemcpyAsync(hE.data(), E, m * sizeof(int), hipMemcpyDeviceToHost, s));
-    CA_HIP(hipMemcpyAsync(hF.data(), F, n * sizeof(int), hipMemcpyDeviceToHost, s));
-    CA_HIP(hipMemcpyAsync(hkA.data(), ksqA, k * sizeof(double), hipMemcpyDeviceToHost, s));
-    CA_HIP(hipMemcpyAsync(hkB.data(), ksqB, k * sizeof(double), hipMemcpyDeviceToHost, s));
-    CA_HIP(hipMemcpyAsync(hrA.data(), rsqA, m * sizeof(double), hipMemcpyDeviceToHost, s));
-    CA_HIP(hipMemcpyAsync(hrB.data(), rsqB, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    // One read-back of the contiguous span ksqA .. F (each small copy costs
+    // tens of microseconds of fixed overhead, which several copies paid in
+    // turn while the GPU waited).
+    const size_t span = (size_t)(2 * Kp + Mp + Np + 1) * 8 + (size_t)(Mp + Np) * 4;
+    std::vector<double> hbuf((span + 7) / 8);
+    CA_HIP(hipMemcpyAsync(hbuf.data(), ksqA, span, hipMemcpyDeviceToHost, s));
     CA_HIP(hipStreamSynchronize(s));
+    const double *hb = hbuf.data();
+    const std::vector<double> hkA(hb, hb + k), hkB(hb + Kp, hb + Kp + k);
+    const std::vector<double> hrA(hb + 2 * Kp, hb + 2 * Kp + m), hrB(hb + 2 * Kp + Mp, hb + 2 * Kp + Mp + n);
+    int nonfinite;
+    std::memcpy(&nonfinite, hb + 2 * Kp + Mp + Np, sizeof(int));
+    const int *hEp = reinterpret_cast<const int *>(hb + 2 * Kp + Mp + Np + 1);
+    const std::vector<int> hE(hEp, hEp + m), hF(hEp + Mp, hEp + Mp + n);
     if (nonfinite) {
         // Inf / NaN cannot be split; the FP64 MFMA DGEMM propagates them as IEEE does.
         if (info) info->method = -1;
