@@ -396,7 +396,13 @@ __global__ __launch_bounds__(256) void gz_products(const OzProblem *probs, int n
     // the descriptors are followed by their int64 tile_start values: the
     // search walks that compact array
     const int64_t *starts = reinterpret_cast<const int64_t *>(probs + nprob);
-    for (int64_t item = blockIdx.x; item < total; item += gridDim.x) {
+    for (int64_t v = blockIdx.x; v < total; v += gridDim.x) {
+        // XCD-local chunks of 8 consecutive tiles (one problem and modulus,
+        // sharing residue panels) on the XCD of workgroup v, as in the FP64
+        // grouped kernels; bijective on [0, total)
+        constexpr int64_t NX = 8, CH = 8, span = NX * CH;
+        const int64_t item =
+            v >= total / span * span ? v : ((v / NX / CH) * NX + v % NX) * CH + (v / NX) % CH;
         int lo = 0, hi = nprob - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) / 2;
