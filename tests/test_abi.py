@@ -73,3 +73,19 @@ def test_header_is_plain_c():
     r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), "-x", "c",
                         "-fsyntax-only", "-"], input=src, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_python_constants_match_the_header():
+    # The enum values the Python binding passes through the ABI are the ones
+    # the header declares (compiled and printed by the C compiler).
+    names = ["COSMA_AMD_OZAKI_MODULI", "COSMA_AMD_OZAKI_SLICES", "COSMA_AMD_OZAKI_MODULI_BF16"]
+    src = "#include <stdio.h>\n#include \"cosma_amd.h\"\nint main(void){" + "".join(
+        f'printf("%d\\n", (int){n});' for n in names) + "return 0;}\n"
+    import tempfile
+    with tempfile.TemporaryDirectory() as tmp:
+        exe = os.path.join(tmp, "enum")
+        r = subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), "-x", "c", "-", "-o", exe],
+                           input=src, capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+        values = [int(v) for v in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
+    assert values == [ca.OZAKI_MODULI, ca.OZAKI_SLICES, ca.OZAKI_MODULI_BF16]
