@@ -383,13 +383,17 @@ __global__ __launch_bounds__(256) void gz_split(const OzProblem *probs, const in
 }
 
 // The INT8 products of every problem and modulus: persistent workgroups walk
-// the 64 x 64 output tiles (problems sorted by decreasing cost), 2 x 2 waves of
-// 32 x 32 (2 x 2 v_mfma_i32_16x16x64_i8 each), k-step 64 double-buffered
-// through LDS, exact INT32 accumulation, reduced modulo the tile's modulus
-// into symmetric int8 residues.
+// the 64 x 64 output tiles (problems sorted by decreasing cost) and compute
+// every modulus of a tile in turn, 2 x 2 waves of 32 x 32 (2 x 2
+// v_mfma_i32_16x16x64_i8 each), k-step 64 double-buffered through LDS, exact
+// INT32 accumulation, reduced modulo the modulus into symmetric int8
+// residues.  The k-steps of all moduli of a tile form one pipeline, so the
+// first step of modulus l + 1 is loaded while modulus l finishes and writes
+// its residues, and the tile's problem is found once for all its moduli.
 __global__ __launch_bounds__(256) void gz_products(const OzProblem *probs, int nprob, int64_t total, GzModuli md) {
     __shared__ __attribute__((aligned(16))) int8_t sa[2][GZ_TILE * GZ_ROW];
     __shared__ __attribute__((aligned(16))) int8_t sb[2][GZ_TILE * GZ_ROW];
+    __shared__ __attribute__((aligned(16))) uint8_t stg[GZ_TILE * GZ_ROW];   // residue staging
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = wave & 1, wn = wave >> 1, g = lane >> 4, li = lane & 15;
     const int lrow = t >> 2, lchunk = t & 3;   // this thread's 16 bytes of a 64 x 64-byte step
@@ -397,9 +401,9 @@ __global__ __launch_bounds__(256) void gz_products(const OzProblem *probs, int n
     // search walks that compact array
     const int64_t *starts = reinterpret_cast<const int64_t *>(probs + nprob);
     for (int64_t v = blockIdx.x; v < total; v += gridDim.x) {
-        // XCD-local chunks of 8 consecutive tiles (one problem and modulus,
-        // sharing residue panels) on the XCD of workgroup v, as in the FP64
-        // grouped kernels; bijective on [0, total)
+        // XCD-local chunks of 8 consecutive tiles (one problem, sharing
+        // residue panels) on the XCD of workgroup v, as in the FP64 grouped
+        // kernels; bijective on [0, total)
         constexpr int64_t NX = 8, CH = 8, span = NX * CH;
         const int64_t item =
             v >= total / span * span ? v : ((v / NX / CH) * NX + v % NX) * CH + (v / NX) % CH;
@@ -410,30 +414,32 @@ __global__ __launch_bounds__(256) void gz_products(const OzProblem *probs, int n
             else hi = mid - 1;
         }
         const OzProblem &q = probs[lo];
-        int64_t local = item - q.tile_start;
-        const int64_t tm = q.Mp / GZ_TILE, per = tm * (q.Np / GZ_TILE);
-        const int l = (int)(local / per);
-        local -= (int64_t)l * per;
+        const int64_t local = item - q.tile_start;
+        const int64_t tm = q.Mp / GZ_TILE;
         const int64_t m0 = (local % tm) * GZ_TILE, n0 = (local / tm) * GZ_TILE;
-        const int8_t *A = q.ares + (int64_t)l * q.Mp * q.Kp + (m0 + lrow) * q.Kp + 16 * lchunk;
-        const int8_t *B = q.bres + (int64_t)l * q.Np * q.Kp + (n0 + lrow) * q.Kp + 16 * lchunk;
-        const int nsteps = (int)(q.Kp / 64);
+        const int8_t *A0 = q.ares + (m0 + lrow) * q.Kp + 16 * lchunk;
+        const int8_t *B0 = q.bres + (n0 + lrow) * q.Kp + 16 * lchunk;
+        const int64_t aplane = q.Mp * q.Kp, bplane = q.Np * q.Kp;
+        const int nsteps = (int)(q.Kp / 64), total_steps = md.count * nsteps;
+        auto load = [&](int f, i32x4 &xa, i32x4 &xb) {
+            const int l = f / nsteps, s = f - l * nsteps;
+            xa = *reinterpret_cast<const i32x4 *>(A0 + (int64_t)l * aplane + 64 * s);
+            xb = *reinterpret_cast<const i32x4 *>(B0 + (int64_t)l * bplane + 64 * s);
+        };
         i32x4 acc[2][2];
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
             for (int j = 0; j < 2; ++j) acc[i][j] = i32x4{0, 0, 0, 0};
         __syncthreads();   // the previous tile's fragment reads are done
-        i32x4 ra = *reinterpret_cast<const i32x4 *>(A), rb = *reinterpret_cast<const i32x4 *>(B);
-        for (int s = 0; s < nsteps; ++s) {
-            const int st = s & 1;
+        i32x4 ra, rb;
+        load(0, ra, rb);
+        for (int f = 0; f < total_steps; ++f) {
+            const int st = f & 1;
             *reinterpret_cast<i32x4 *>(sa[st] + lrow * GZ_ROW + 16 * lchunk) = ra;
             *reinterpret_cast<i32x4 *>(sb[st] + lrow * GZ_ROW + 16 * lchunk) = rb;
             __syncthreads();
-            if (s + 1 < nsteps) {
-                ra = *reinterpret_cast<const i32x4 *>(A + 64 * (s + 1));
-                rb = *reinterpret_cast<const i32x4 *>(B + 64 * (s + 1));
-            }
+            if (f + 1 < total_steps) load(f + 1, ra, rb);
             i32x4 fa[2], fb[2];
 #pragma unroll
             for (int i = 0; i < 2; ++i)
@@ -446,37 +452,40 @@ __global__ __launch_bounds__(256) void gz_products(const OzProblem *probs, int n
 #pragma unroll
                 for (int j = 0; j < 2; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[i], fb[j], acc[i][j], 0, 0, 0);
-        }
-        // lane 16 g + li holds rows 4g .. 4g + 3 of column li of each 16 x 16
-        // block.  The 64 x 64 residues go through LDS (column c of the tile at
-        // bytes [80 c, 80 c + 64): the 4-byte writes of a wave hit 64
-        // different banks) and leave as 16-byte pieces, four per column.
-        const int mod = md.m[l];
-        const float inv = md.inv_mf[l];
-        uint8_t *tile = reinterpret_cast<uint8_t *>(&sa[0][0]);
-        __syncthreads();   // every wave's last fragment reads are done
+            if ((f + 1) % nsteps) continue;
+            // modulus l done.  Lane 16 g + li holds rows 4g .. 4g + 3 of
+            // column li of each 16 x 16 block.  The 64 x 64 residues go
+            // through the staging tile (column c at bytes [80 c, 80 c + 64):
+            // the 4-byte writes of a wave hit 64 different banks) and leave
+            // as 16-byte pieces, four per column.  The staging writes follow
+            // the previous modulus's staging reads by at least one k-step
+            // barrier.
+            const int l = f / nsteps;
+            const int mod = md.m[l];
+            const float inv = md.inv_mf[l];
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+            for (int i = 0; i < 2; ++i)
 #pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                const int rt = wm * 32 + 16 * i + 4 * g, ct = wn * 32 + 16 * j + li;
-                uint32_t w = 0;
+                for (int j = 0; j < 2; ++j) {
+                    const int rt = wm * 32 + 16 * i + 4 * g, ct = wn * 32 + 16 * j + li;
+                    uint32_t w = 0;
 #pragma unroll
-                for (int e = 0; e < 4; ++e) w |= (uint32_t)(uint8_t)(int8_t)sym_mod(acc[i][j][e], mod, inv) << (8 * e);
-                *reinterpret_cast<uint32_t *>(tile + ct * GZ_ROW + rt) = w;
-            }
-        __syncthreads();
-        {
+                    for (int e = 0; e < 4; ++e)
+                        w |= (uint32_t)(uint8_t)(int8_t)sym_mod(acc[i][j][e], mod, inv) << (8 * e);
+                    *reinterpret_cast<uint32_t *>(stg + ct * GZ_ROW + rt) = w;
+                    acc[i][j] = i32x4{0, 0, 0, 0};
+                }
+            __syncthreads();
             const int ct = t >> 2, c = t & 3;
             const int64_t col = n0 + ct, row = m0 + 16 * c;
             if (col < q.n && row < q.m) {
-                const i32x4 v = *reinterpret_cast<const i32x4 *>(tile + ct * GZ_ROW + 16 * c);
+                const i32x4 val = *reinterpret_cast<const i32x4 *>(stg + ct * GZ_ROW + 16 * c);
                 int8_t *o = q.ores + (int64_t)l * q.oplane + row + col * q.ldr;
                 if (row + 16 <= q.m) {
-                    *reinterpret_cast<i32x4 *>(o) = v;
+                    *reinterpret_cast<i32x4 *>(o) = val;
                 } else {
                     union { i32x4 q; int8_t b[16]; } u;
-                    u.q = v;
+                    u.q = val;
                     for (int e = 0; e < 16 && row + e < q.m; ++e) o[e] = u.b[e];
                 }
             }
@@ -856,7 +865,7 @@ int grouped_ozaki_create(cosma_amd_handle_t h, const cosma_amd_gemm_desc *d, int
         g.bres = base + (size_t)moduli * g.Mp * g.Kp;
         g.ores = g.bres + (size_t)moduli * g.Np * g.Kp;
         g.oplane = up(g.ldr * g.n, 16);
-        tiles += (g.Mp / GZ_TILE) * (g.Np / GZ_TILE) * moduli;
+        tiles += (g.Mp / GZ_TILE) * (g.Np / GZ_TILE);   // each tile computes every modulus
     }
     p->total_tiles = tiles;
     if ((rc = upload())) return fail(rc);
