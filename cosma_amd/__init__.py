@@ -225,6 +225,9 @@ class Grouped:
     """A grouped-GEMM plan over device-resident problems.
 
     problems: iterable of (transa, transb, m, n, k, alpha, a_ptr, lda, b_ptr, ldb, beta, c_ptr, ldc).
+    tile_config: 0 = tuned choice for these shapes if known in the process, else 1; 1 = 64x64,
+    2 = 128x64, 3 = 128x128 (LDS-staged, four waves); 4 = 64x64, 5 = 32x32 (one wave per tile,
+    fragments straight from global memory).  tune() times them all with several grids.
     """
 
     def __init__(self, handle, problems, tile_config=0):
