@@ -100,7 +100,45 @@ def build_test_driver(verbose=False):
     return TEST_DRIVER
 
 
+CPP_TESTS = os.path.join(ROOT, "tests", "cpp")
+STUB_LIB = os.path.join(CPP_TESTS, "libstub_scalapack.so")
+# interposition test executables: name -> libraries in link (= lookup) order
+INTERPOSE_LINKS = {
+    "interpose_first": ["libcosma_amd_pxgemm.so", "libstub_scalapack.so"],
+    "cosma_first": ["libcosma_amd.so", "libcosma_amd_pxgemm.so", "libstub_scalapack.so"],
+    "stub_only": ["libstub_scalapack.so"],
+    "no_stub": ["libcosma_amd_pxgemm.so"],
+}
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+
+
+def build_interpose_tests(verbose=False):
+    """Builds the stand-in ScaLAPACK (tests/cpp/libstub_scalapack.so) and the
+    interposition test driver linked in each order of INTERPOSE_LINKS."""
+    _run([CXX, *HOST, "-shared", "-o", STUB_LIB + ".tmp", os.path.join(CPP_TESTS, "stub_scalapack.cpp")])
+    os.replace(STUB_LIB + ".tmp", STUB_LIB)
+    src = os.path.join(CPP_TESTS, "interpose_test.cpp")
+    out = {}
+    for name, libs in INTERPOSE_LINKS.items():
+        exe = os.path.join(CPP_TESTS, "interpose_" + name)
+        cmd = [CXX, *HOST, "-o", exe + ".tmp", src, "-L", HERE, "-L", CPP_TESTS, "-Wl,--no-as-needed"]
+        cmd += ["-l:" + lib for lib in libs]
+        cmd += ["-Wl,--as-needed", "-ldl", "-Wl,-rpath,$ORIGIN/../../cosma_amd:$ORIGIN"]
+        _run(cmd)
+        os.replace(exe + ".tmp", exe)
+        out[name] = exe
+    if verbose:
+        print(f"built the interposition tests in {CPP_TESTS}")
+    return out
+
+
 if __name__ == "__main__":
     build(verbose=True)
     build_test_driver(verbose=True)
+    build_interpose_tests(verbose=True)
     sys.exit(0)

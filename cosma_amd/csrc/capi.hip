@@ -169,6 +169,15 @@ int cosma_amd_dgemm_ozaki(cosma_amd_handle_t h, char transa, char transb,
         CA_CHECK(ld_ok(lda, ta == 'N' ? m : k), "dgemm_ozaki: lda=%lld too small", (long long)lda);
         CA_CHECK(ld_ok(ldb, tb == 'N' ? k : n), "dgemm_ozaki: ldb=%lld too small", (long long)ldb);
         CA_CHECK(a != nullptr && b != nullptr, "dgemm_ozaki: null A or B");
+        // The INT8 / BF16 product kernel addresses a 256-row tile of an operand
+        // plane with 32-bit offsets: 256 rows of round_up(k, 256) * (2 for
+        // BF16) bytes must stay below 2^31.  Checked before any work is queued.
+        const int64_t row_bytes = (k + 255) / 256 * 256 * (method == COSMA_AMD_OZAKI_MODULI_BF16 ? 2 : 1);
+        if (row_bytes * 256 >= ((int64_t)1 << 31)) {
+            set_error("dgemm_ozaki: k = %lld gives operand rows of %lld bytes; the product kernel supports rows "
+                      "below %lld bytes", (long long)k, (long long)row_bytes, (long long)(((int64_t)1 << 31) / 256));
+            return COSMA_AMD_ERR_UNSUPPORTED;
+        }
     }
     CA_HIP(hipSetDevice(h->device));
     return ozaki_dgemm(h, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, method, tolerance, count,

@@ -21,9 +21,9 @@ inline void check(int status, const char *where) {
 // the process, else 1.
 int comm_size(MPI_Comm comm);
 
-// Looks a symbol up in the process: the next definition after this library
-// (next = true) or the first one (next = false); nullptr if absent.
-void *find_symbol(const char *name, bool next);
+// Looks a symbol up in the process (its first definition in the global
+// lookup order); nullptr if absent.
+void *find_symbol(const char *name);
 
 }  // namespace detail
 }  // namespace cosma

@@ -7,6 +7,11 @@
 // (src/cosma/pxgemm.cpp:8-17 and its INTERPOSE_C_VOID definitions).  The
 // symbols are strong here; the reference makes them weak aliases so that a
 // static ScaLAPACK can win at link time instead.
+#ifndef _GNU_SOURCE
+#define _GNU_SOURCE
+#endif
+#include <dlfcn.h>
+
 #include <string>
 
 #include <cosma/cosma_pxgemm.hpp>
@@ -31,10 +36,27 @@ namespace {
 using d_fn = void (*)(D_ARGS);
 using s_fn = void (*)(S_ARGS);
 
-// The definition of `name` after this library, or nullptr.
+// Load address of this library, to recognise a lookup that lands in it.
+const void *self_base() {
+    static const void *base = [] {
+        Dl_info info{};
+        return dladdr(reinterpret_cast<const void *>(&self_base), &info) ? info.dli_fbase : nullptr;
+    }();
+    return base;
+}
+
+// The definition of `name` after this library in the lookup order, or
+// nullptr.  dlsym(RTLD_NEXT) must run here: it searches the objects after
+// the one that calls it, so called from libcosma_amd.so it would start the
+// search in the wrong place (and could return these very functions).
 template <typename F>
 F next_symbol(const char *name) {
-    return reinterpret_cast<F>(cosma::detail::find_symbol(name, true));
+    void *p = dlsym(RTLD_NEXT, name);
+    if (p) {
+        Dl_info info{};
+        if (dladdr(p, &info) && info.dli_fbase == self_base()) p = nullptr;
+    }
+    return reinterpret_cast<F>(p);
 }
 
 void do_pdgemm(const char *name, D_ARGS) {

@@ -9,16 +9,16 @@
 namespace cosma {
 namespace detail {
 
-void *find_symbol(const char *name, bool next) { return dlsym(next ? RTLD_NEXT : RTLD_DEFAULT, name); }
+void *find_symbol(const char *name) { return dlsym(RTLD_DEFAULT, name); }
 
 int comm_size(MPI_Comm comm) {
     // Open MPI's handles are pointers, not the integer handles this
     // interface was built with: nothing can be asked then.
-    if (find_symbol("ompi_mpi_comm_world", false)) return 1;
+    if (find_symbol("ompi_mpi_comm_world")) return 1;
     using initialized_fn = int (*)(int *);
     using size_fn = int (*)(MPI_Comm, int *);
-    auto initialized = reinterpret_cast<initialized_fn>(find_symbol("MPI_Initialized", false));
-    auto size = reinterpret_cast<size_fn>(find_symbol("MPI_Comm_size", false));
+    auto initialized = reinterpret_cast<initialized_fn>(find_symbol("MPI_Initialized"));
+    auto size = reinterpret_cast<size_fn>(find_symbol("MPI_Comm_size"));
     if (!initialized || !size) return 1;
     int flag = 0;
     if (initialized(&flag) != 0 || !flag) return 1;

@@ -22,7 +22,7 @@ namespace scalapack {
 grid_info get_grid_info(int ctxt) {
     grid_info g;
     using gridinfo_fn = void (*)(int, int *, int *, int *, int *);
-    auto f = reinterpret_cast<gridinfo_fn>(detail::find_symbol("Cblacs_gridinfo", false));
+    auto f = reinterpret_cast<gridinfo_fn>(detail::find_symbol("Cblacs_gridinfo"));
     if (f) f(ctxt, &g.nprow, &g.npcol, &g.myrow, &g.mycol);
     return g;
 }
@@ -165,7 +165,7 @@ using sgemm_fn = void (*)(const char *, const char *, const int *, const int *, 
 // ScaLAPACK's own definition of `name` (trailing-underscore spelling), if loaded.
 template <typename F>
 F scalapack_symbol(const char *name) {
-    return reinterpret_cast<F>(cosma::detail::find_symbol(name, false));
+    return reinterpret_cast<F>(cosma::detail::find_symbol(name));
 }
 
 void prefixed_pdgemm(const char *ta, const char *tb, const int *m, const int *n, const int *k, const double *alpha,

@@ -226,6 +226,17 @@ int download(cosma_amd_handle_t h, double *dst, int64_t dld, const double *src, 
     return COSMA_AMD_OK;
 }
 
+// After a failed call: waits for the handle's streams (ignoring their
+// errors) and drops every pending copy-out of the D2H staging slots, so that
+// no later call copies a stale tile into a caller buffer of this one.
+void abandon(cosma_amd_handle_t h) {
+    (void)hipStreamSynchronize(h->h2d);
+    for (auto s : h->streams) (void)hipStreamSynchronize(s);
+    (void)hipStreamSynchronize(h->d2h);
+    (void)hipGetLastError();
+    for (auto &s : h->d2h_slots) s.out = nullptr;
+}
+
 int sync_all(cosma_amd_handle_t h) {
     CA_HIP(hipStreamSynchronize(h->h2d));
     for (auto s : h->streams) CA_HIP(hipStreamSynchronize(s));
@@ -551,9 +562,9 @@ static int stream_bounded(cosma_amd_handle_t h, const Operand &A, const Operand 
     return COSMA_AMD_OK;
 }
 
-int stream_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k,
-                 double alpha, const double *a, int64_t lda, const double *b, int64_t ldb,
-                 double beta, double *c, int64_t ldc, bool copy_c_back) {
+static int stream_dgemm_body(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k,
+                             double alpha, const double *a, int64_t lda, const double *b, int64_t ldb,
+                             double beta, double *c, int64_t ldc, bool copy_c_back) {
     if (m == 0 || n == 0) return COSMA_AMD_OK;
     const bool scale_only = (k == 0 || alpha == 0.0);
     Operand A{a, lda, scale_only ? DEVICE : classify(a), ta, ta == 'N' ? m : k, ta == 'N' ? k : m};
@@ -604,6 +615,15 @@ int stream_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, i
                                        tm, tn, tk, nTm, nTn, nTk);
     if (rc) return rc;
     return sync_all(h);
+}
+
+int stream_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k,
+                 double alpha, const double *a, int64_t lda, const double *b, int64_t ldb,
+                 double beta, double *c, int64_t ldc, bool copy_c_back) {
+    for (auto &s : h->d2h_slots) s.out = nullptr;   // nothing of an earlier call is pending
+    const int rc = stream_dgemm_body(h, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, copy_c_back);
+    if (rc) abandon(h);
+    return rc;
 }
 
 // ------------------------------------------------------ grid-layout multiply
@@ -682,9 +702,9 @@ int gather_blocks(cosma_amd_handle_t h, const cosma_amd_block *blk, int64_t nb, 
 
 }  // namespace
 
-int layout_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha,
-                 const cosma_amd_block *A, int64_t na, const cosma_amd_block *B, int64_t nb, double beta,
-                 const cosma_amd_block *C, int64_t nc) {
+static int layout_dgemm_body(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k,
+                             double alpha, const cosma_amd_block *A, int64_t na, const cosma_amd_block *B,
+                             int64_t nb, double beta, const cosma_amd_block *C, int64_t nc) {
     const int64_t arows = ta == 'N' ? m : k, acols = ta == 'N' ? k : m;
     const int64_t brows = tb == 'N' ? k : n, bcols = tb == 'N' ? n : k;
     int rc;
@@ -782,6 +802,15 @@ int layout_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, i
         }
     }
     return sync_all(h);
+}
+
+int layout_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha,
+                 const cosma_amd_block *A, int64_t na, const cosma_amd_block *B, int64_t nb, double beta,
+                 const cosma_amd_block *C, int64_t nc) {
+    for (auto &s : h->d2h_slots) s.out = nullptr;   // nothing of an earlier call is pending
+    const int rc = layout_dgemm_body(h, ta, tb, m, n, k, alpha, A, na, B, nb, beta, C, nc);
+    if (rc) abandon(h);
+    return rc;
 }
 
 }  // namespace cosma_amd
