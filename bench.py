@@ -118,6 +118,31 @@ def link_bandwidth(torch, nbytes=1 << 31):
     return 3 * nbytes / (time.perf_counter() - t0) / 1e9
 
 
+def streamed_error(h, ta, tb, m, n, k, alpha, beta, host, C0_seed):
+    """One more call from the seeded C0, then the relative error of rows 0 and
+    m-1 and column n/2 of C against the double-double oracle (all k terms)."""
+    import torch
+    import cosma_amd as ca
+    from oracle.sampled import dd_lines, lines_error
+    C = host["C"]                                   # torch (n, m): row j = column j of C
+    cols_dev = max(1, (1 << 28) // m)
+    tmp = torch.empty((min(cols_dev, n), m), dtype=torch.float64, device="cuda")
+    for c0 in range(0, n, cols_dev):
+        cc = min(cols_dev, n - c0)
+        ca.fill_uniform(tmp.data_ptr(), m, cc, m, C0_seed, offset=c0 * m)
+        C[c0:c0 + cc].copy_(tmp[:cc])
+    torch.cuda.synchronize()
+    del tmp
+    rows, cols = (0, m - 1), (n // 2,)
+    c0r = {i: C[:, i].numpy().copy() for i in rows}
+    c0c = {j: C[j, :].numpy().copy() for j in cols}
+    h.dgemm_host(ta, tb, m, n, k, alpha, host["A"].data_ptr(), k, host["B"].data_ptr(), k, beta, C.data_ptr(), m)
+    r, c = dd_lines(ta, tb, m, n, k, host["A"].data_ptr(), k, host["B"].data_ptr(), k, rows, cols)
+    return lines_error({i: C[:, i].numpy() for i in rows}, {j: C[j, :].numpy() for j in cols},
+                       {i: alpha * r[i] + beta * c0r[i] for i in rows},
+                       {j: alpha * c[j] + beta * c0c[j] for j in cols})
+
+
 def streamed(args):
     """Host-resident workloads through the host-streaming DGEMM."""
     import torch
@@ -162,6 +187,7 @@ def streamed(args):
         if i >= args.warmup:
             times.append(dt)
     t = sum(times) / len(times)
+    err = None if args.no_check else streamed_error(h, ta, tb, m, n, k, alpha, beta, host, C0_seed)
     flops = 2.0 * m * n * k
     h2d = 8.0 * (m * k + k * n + (m * n if beta != 0 else 0))
     d2h = 8.0 * m * n
@@ -174,6 +200,8 @@ def streamed(args):
         "data": "synthetic (seeded SplitMix64 uniform[-1,1))",
         "config": {"workload": desc, "m": m, "n": n, "k": k, "tiles": tiles, "schedule":
                    {0: "device", 1: "resident panels", 2: "bounded ring"}[h.last_mode()]},
+        "error": {"rel_fro_sampled_lines_vs_double_double": err, "bound_4k_u": 4 * k * 2.0 ** -53,
+                  "lines": "rows 0, m-1 and column n/2 of C, each with all k terms"},
         "roofline": {"bound": "host-link" if t_link > t_mfma else "mfma",
                      "achieved": round(h2d / t / 1e9, 2), "peak": round(link, 2), "unit": "GB/s",
                      "frac": round(max(t_link, t_mfma) / t, 4), "traffic": h2d + d2h,
@@ -396,6 +424,14 @@ def reorder(args):
     print(json.dumps(out), flush=True)
 
 
+def batch_dims(count=4096, seed=5):
+    """m, n, k of the batched workload: each round(exp(U[ln 16, ln 512]))
+    clamped to [16, 512], numpy default_rng(seed); shape (count, 3)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    return np.clip(np.rint(np.exp(rng.uniform(np.log(16), np.log(512), (count, 3)))), 16, 512).astype(np.int64)
+
+
 def batched(args, count=4096, seed=5):
     """4096 independent C = A B problems (NN, alpha = 1, beta = 0), m, n, k each
     round(exp(U[ln 16, ln 512])) clamped to [16, 512] (numpy default_rng(5)),
@@ -405,8 +441,8 @@ def batched(args, count=4096, seed=5):
     import numpy as np
     import torch
     import cosma_amd as ca
-    rng = np.random.default_rng(seed)
-    dims = np.clip(np.rint(np.exp(rng.uniform(np.log(16), np.log(512), (count, 3)))), 16, 512).astype(np.int64)
+    dims = batch_dims(count, seed)
+
     def offsets(sizes):
         sizes = (sizes + 1) // 2 * 2                      # keep every operand 16-byte aligned
         return np.concatenate([[0], np.cumsum(sizes)])

@@ -38,6 +38,7 @@ class CosmaAmdError(RuntimeError):
 OZAKI_MODULI = 0
 OZAKI_SLICES = 1
 OZAKI_MODULI_BF16 = 2
+OZAKI_STRICT = 0x100   # flag: the tolerance must hold for the rigorous bound
 
 
 class Block(ctypes.Structure):
@@ -63,7 +64,9 @@ class GemmDesc(ctypes.Structure):
 
 class OzakiInfo(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int), ("count", ctypes.c_int), ("bits", ctypes.c_int),
-                ("int8_products", ctypes.c_int64), ("error_bound", ctypes.c_double)]
+                ("int8_products", ctypes.c_int64), ("error_bound", ctypes.c_double),
+                ("error_estimate", ctypes.c_double), ("tolerance_met", ctypes.c_int), ("passes", ctypes.c_int),
+                ("product_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -158,8 +158,9 @@ int cosma_amd_dgemm_ozaki(cosma_amd_handle_t h, char transa, char transb,
     const char ta = norm_trans(transa), tb = norm_trans(transb);
     CA_CHECK(ta != 0, "dgemm_ozaki: invalid transa '%c'", transa);
     CA_CHECK(tb != 0, "dgemm_ozaki: invalid transb '%c'", transb);
-    CA_CHECK(method == COSMA_AMD_OZAKI_MODULI || method == COSMA_AMD_OZAKI_SLICES ||
-                 method == COSMA_AMD_OZAKI_MODULI_BF16,
+    const int base_method = method & ~COSMA_AMD_OZAKI_STRICT;
+    CA_CHECK(base_method == COSMA_AMD_OZAKI_MODULI || base_method == COSMA_AMD_OZAKI_SLICES ||
+                 base_method == COSMA_AMD_OZAKI_MODULI_BF16,
              "dgemm_ozaki: unknown method %d", method);
     CA_CHECK(m >= 0 && n >= 0 && k >= 0, "dgemm_ozaki: negative extent");
     if (m == 0 || n == 0) return COSMA_AMD_OK;
@@ -172,7 +173,7 @@ int cosma_amd_dgemm_ozaki(cosma_amd_handle_t h, char transa, char transb,
         // The INT8 / BF16 product kernel addresses a 256-row tile of an operand
         // plane with 32-bit offsets: 256 rows of round_up(k, 256) * (2 for
         // BF16) bytes must stay below 2^31.  Checked before any work is queued.
-        const int64_t row_bytes = (k + 255) / 256 * 256 * (method == COSMA_AMD_OZAKI_MODULI_BF16 ? 2 : 1);
+        const int64_t row_bytes = (k + 255) / 256 * 256 * (base_method == COSMA_AMD_OZAKI_MODULI_BF16 ? 2 : 1);
         if (row_bytes * 256 >= ((int64_t)1 << 31)) {
             set_error("dgemm_ozaki: k = %lld gives operand rows of %lld bytes; the product kernel supports rows "
                       "below %lld bytes", (long long)k, (long long)row_bytes, (long long)(((int64_t)1 << 31) / 256));

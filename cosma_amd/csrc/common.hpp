@@ -84,6 +84,7 @@ struct cosma_amd_handle_s {
     cosma_amd::DevBuf dC0;              // ring of incoming C tiles (deferred beta * C)
     cosma_amd::DevBuf ozaki_ws;         // split-precision workspace
     cosma_amd::DevBuf ozaki_aux;        // split-precision row statistics and exponents
+    hipEvent_t ozaki_ev[2] = {nullptr, nullptr};   // around the split-precision product launch
     std::vector<hipEvent_t> events;     // event pool
     size_t events_used = 0;
     std::vector<cosma_amd::StagingSlot> h2d_slots, d2h_slots;

@@ -67,7 +67,8 @@ constexpr int STATS_KC = 8;   // 64-wide k chunks per workgroup (contiguous vect
 
 template <bool CONTIG>
 __global__ __launch_bounds__(256) void stats_kernel(const double *__restrict__ x, int64_t ld, int64_t R, int64_t K,
-                                                    unsigned long long *maxbits, double *ksq, double *rsq) {
+                                                    unsigned long long *maxbits, double *ksq, double *rsq,
+                                                    double *rabs) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     constexpr int PER_WAVE = STATS_SPAN / 4, U = 8;
     if (CONTIG) {
@@ -82,11 +83,12 @@ __global__ __launch_bounds__(256) void stats_kernel(const double *__restrict__ x
         for (int c = 0; c < STATS_KC; ++c) ss[c] = 0.0;
         for (int q0 = 0; q0 < PER_WAVE; q0 += U) {
             unsigned long long b[U];
-            double q[U];
+            double q[U], l1[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 b[u] = 0;
                 q[u] = 0.0;
+                l1[u] = 0.0;
             }
 #pragma unroll
             for (int c = 0; c < STATS_KC; ++c) {
@@ -101,6 +103,7 @@ __global__ __launch_bounds__(256) void stats_kernel(const double *__restrict__ x
                 for (int u = 0; u < U; ++u) {
                     ss[c] = fma(v[u], v[u], ss[c]);
                     q[u] = fma(v[u], v[u], q[u]);
+                    l1[u] += fabs(v[u]);
                     const unsigned long long bb = __double_as_longlong(fabs(v[u]));
                     b[u] = bb > b[u] ? bb : b[u];
                 }
@@ -114,18 +117,21 @@ __global__ __launch_bounds__(256) void stats_kernel(const double *__restrict__ x
                     const unsigned long long o = __shfl_xor(b[u], off);
                     b[u] = o > b[u] ? o : b[u];
                     q[u] += __shfl_xor(q[u], off);
+                    l1[u] += __shfl_xor(l1[u], off);
                 }
             }
             if (lane < U && rb + q0 + lane < R) {
                 unsigned long long mine = b[0];
-                double qs = q[0];
+                double qs = q[0], as = l1[0];
 #pragma unroll
                 for (int u = 1; u < U; ++u) {
                     mine = lane == u ? b[u] : mine;
                     qs = lane == u ? q[u] : qs;
+                    as = lane == u ? l1[u] : as;
                 }
                 atomicMax(&maxbits[rb + q0 + lane], mine);
                 atomicAdd(&rsq[rb + q0 + lane], qs);
+                atomicAdd(&rabs[rb + q0 + lane], as);
             }
         }
 #pragma unroll
@@ -136,7 +142,7 @@ __global__ __launch_bounds__(256) void stats_kernel(const double *__restrict__ x
         const int64_t kb = (int64_t)blockIdx.y * STATS_SPAN + wave * PER_WAVE;
         const bool rin = r < R;
         unsigned long long best = 0;
-        double rs = 0.0;
+        double rs = 0.0, ra = 0.0;
         for (int q0 = 0; q0 < PER_WAVE; q0 += U) {
             double v[U];
 #pragma unroll
@@ -151,6 +157,7 @@ __global__ __launch_bounds__(256) void stats_kernel(const double *__restrict__ x
                 best = b > best ? b : best;
                 sq[u] = v[u] * v[u];
                 rs += sq[u];
+                ra += fabs(v[u]);
             }
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) {
@@ -167,6 +174,7 @@ __global__ __launch_bounds__(256) void stats_kernel(const double *__restrict__ x
         if (rin) {
             atomicMax(&maxbits[r], best);
             atomicAdd(&rsq[r], rs);
+            atomicAdd(&rabs[r], ra);
         }
     }
 }
@@ -277,16 +285,41 @@ __global__ __launch_bounds__(256) void split_strided(SplitArgs a) {
 // FP64 integer arithmetic (see crt_value).
 constexpr int CRT_ROWS = 4;
 constexpr int64_t PLANE_PAD = 3 * 1024;   // bytes; a multiple of 16, not of a large power of two
+// The epilogues also return the Frobenius norm of the product X = op(A)op(B)
+// as computed (before alpha and beta), for the check of the error against
+// the tolerance: sum of (X_ij 2^-shift)^2 in NORM_SLOTS partial sums (one
+// atomic add per workgroup).  Epilogue modes:
+constexpr int EPI_WRITE = 0;        // C <- alpha X + beta C
+constexpr int EPI_WRITE_NORM = 1;   // the same, and the norm of X
+constexpr int EPI_NORM = 2;         // the norm of X only (C untouched)
+constexpr int NORM_SLOTS = 64;
+
+// Adds v (one per thread) over the workgroup into nrm[slot] (all threads of
+// the workgroup must call it).
+__device__ __forceinline__ void block_norm_add(double v, double *nrm) {
+    __shared__ double part[8];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) part[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += part[w];
+        atomicAdd(&nrm[(blockIdx.x + blockIdx.y * gridDim.x) % NORM_SLOTS], t);
+    }
+}
+
 // NL limbs of 41 bits: 3 while M < 2^122 (up to 15 moduli), else 4.
-template <int NL>
+template <int NL, int MODE>
 __global__ __launch_bounds__(256) void crt_epilogue(CrtConst cc, const int8_t *__restrict__ res, int64_t ldr,
                                                     int64_t res_stride, int64_t M, int64_t N, int P,
                                                     const int *__restrict__ E, const int *__restrict__ F,
                                                     double alpha, double beta, double *__restrict__ C,
-                                                    int64_t ldc) {
+                                                    int64_t ldc, int shift, double *__restrict__ nrm) {
     const int64_t i0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * CRT_ROWS;
-    if (i0 >= M) return;
-    for (int64_t j = blockIdx.y; j < N; j += gridDim.y) {
+    double ss = 0.0;
+    for (int64_t j = blockIdx.y; i0 < M && j < N; j += gridDim.y) {
         const int8_t *rp = res + i0 + j * ldr;
         // S = sum_l c_l W_l limb by limb
         double s[CRT_ROWS][NL];
@@ -310,24 +343,43 @@ __global__ __launch_bounds__(256) void crt_epilogue(CrtConst cc, const int8_t *_
             const int64_t i = i0 + e;
             if (i >= M) break;
             const double v = crt_value<NL>(s[e], cc);
-            const double val = alpha * ldexp(v, E[i] + fj - 2 * P);
+            if (MODE != EPI_WRITE) {
+                const double x = ldexp(v, E[i] + fj - 2 * P - shift);
+                ss = fma(x, x, ss);
+            }
+            if (MODE != EPI_NORM) {
+                const double val = alpha * ldexp(v, E[i] + fj - 2 * P);
+                double *cp = C + i + j * ldc;
+                *cp = (beta == 0.0) ? val : fma(beta, *cp, val);
+            }
+        }
+    }
+    if (MODE != EPI_WRITE) block_norm_add(ss, nrm);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void slices_epilogue(const double *__restrict__ acc, int64_t lda, int64_t M,
+                                                       int64_t N, const int *__restrict__ E,
+                                                       const int *__restrict__ F, double alpha, double beta,
+                                                       double *__restrict__ C, int64_t ldc, int shift,
+                                                       double *__restrict__ nrm) {
+    const int64_t total = M * N;
+    double ss = 0.0;
+    for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = idx / M, i = idx - j * M;
+        const double v = acc[i + j * lda];
+        if (MODE != EPI_WRITE) {
+            const double x = ldexp(v, E[i] + F[j] - shift);
+            ss = fma(x, x, ss);
+        }
+        if (MODE != EPI_NORM) {
+            const double val = alpha * ldexp(v, E[i] + F[j]);
             double *cp = C + i + j * ldc;
             *cp = (beta == 0.0) ? val : fma(beta, *cp, val);
         }
     }
-}
-
-__global__ void slices_epilogue(const double *__restrict__ acc, int64_t lda, int64_t M, int64_t N,
-                                const int *__restrict__ E, const int *__restrict__ F, double alpha, double beta,
-                                double *__restrict__ C, int64_t ldc) {
-    const int64_t total = M * N;
-    for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
-         idx += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t j = idx / M, i = idx - j * M;
-        const double val = alpha * ldexp(acc[i + j * lda], E[i] + F[j]);
-        double *cp = C + i + j * ldc;
-        *cp = (beta == 0.0) ? val : fma(beta, *cp, val);
-    }
+    if (MODE != EPI_WRITE) block_norm_add(ss, nrm);
 }
 
 inline int64_t round_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
@@ -425,27 +477,98 @@ void crt_constants(int n, CrtConst &cc) {
 }
 
 namespace {
-// gamma such that the relative Frobenius error of the product of the P-bit
-// roundings is about gamma * 2^-P (see ozaki_dgemm); +inf if it cannot be
-// formed (overflowed sums), 0 for a zero product.
-double error_gamma(const std::vector<int> &E, const std::vector<int> &F, const std::vector<double> &ksqA,
-                   const std::vector<double> &ksqB) {
+// Error model of the split-precision product X' (what the epilogue computes
+// before alpha and beta) against X = op(A) op(B).  Rounding the entries of
+// row i of op(A) to the grid of spacing 2^(E_i - P) moves each by at most
+// delta_i = 2^(E_i - P - 1) (columns of op(B): eps_j = 2^(F_j - P - 1)), and
+//   X' - X = dA B + A dB + dA dB,
+// so, entry by entry,
+//   |X'_ij - X_ij| <= delta_i ||op(B)(:, j)||_1 + eps_j ||op(A)(i, :)||_1 + k delta_i eps_j,
+// whose Frobenius norm is at most
+//   ||delta|| ||b1|| + ||a1|| ||eps|| + k ||delta|| ||eps||     (a rigorous bound),
+// a1_i, b1_j the 1-norms of the rows of op(A) / columns of op(B).  With
+// independent rounding errors of variance delta^2 / 3 the expected square is
+//   (1/3) (||delta||^2 ||B||_F^2 + ||A||_F^2 ||eps||^2)           (the estimate).
+// Slices: s digits of 7 bits leave delta_i = 2^(E_i - 7s - 1); the products of
+// digit pairs with t + u >= s + 2 that are not computed add at most
+// k D_s 2^(E_i + F_j), D_s = sum_{d = s+2}^{2s} (2s + 1 - d) 64^2 2^-7d (the
+// digits after the leading one lie in [-64, 64]).
+// Everything is held relative to 2^(ea + fb), ea / fb the largest exponents.
+struct ErrModel {
+    bool zero = false;      // a zero operand (every row or column exponent is the zero sentinel)
+    int ea = 0, fb = 0;
+    int64_t k = 0;
+    double sA = 0, sB = 0;  // ||2^(E - ea)||, ||2^(F - fb)||
+    double a1 = 0, b1 = 0;  // ||a1 2^-ea||, ||b1 2^-fb||
+    double fA = 0, fB = 0;  // ||op(A)||_F 2^-ea, ||op(B)||_F 2^-fb
+    double d0 = 0;          // sqrt(sum_l ||op(A)(:, l)||^2 ||op(B)(l, :)||^2) 2^-(ea + fb): the
+                            // size of X the a-priori estimate assumes (independent signs)
+    double gamma = 0;       // a-priori relative estimate = gamma * 2^-P (doubled where used)
+};
+
+ErrModel error_model(const std::vector<int> &E, const std::vector<int> &F, const std::vector<double> &ksqA,
+                     const std::vector<double> &ksqB, const std::vector<double> &l1A,
+                     const std::vector<double> &l1B, int64_t k) {
+    ErrModel em;
+    em.k = k;
     int ea = INT_MIN, fb = INT_MIN;
-    for (int e : E) ea = std::max(ea, e);
-    for (int f : F) fb = std::max(fb, f);
-    if (ea == INT_MIN || fb == INT_MIN || ea == ZERO_EXPONENT || fb == ZERO_EXPONENT) return 0.0;
-    double SA = 0.0, SB = 0.0, NA = 0.0, NB = 0.0, D = 0.0;
-    for (int e : E) SA += std::ldexp(1.0, 2 * (e - ea));
-    for (int f : F) SB += std::ldexp(1.0, 2 * (f - fb));
+    for (int e : E) if (e != ZERO_EXPONENT) ea = std::max(ea, e);
+    for (int f : F) if (f != ZERO_EXPONENT) fb = std::max(fb, f);
+    if (ea == INT_MIN || fb == INT_MIN) {
+        em.zero = true;
+        return em;
+    }
+    em.ea = ea;
+    em.fb = fb;
+    double SA = 0.0, SB = 0.0, NA = 0.0, NB = 0.0, D = 0.0, LA = 0.0, LB = 0.0;
+    for (size_t i = 0; i < E.size(); ++i) {
+        if (E[i] == ZERO_EXPONENT) continue;
+        SA += std::ldexp(1.0, 2 * (E[i] - ea));
+        const double a = std::ldexp(l1A[i], -ea);
+        LA += a * a;
+    }
+    for (size_t j = 0; j < F.size(); ++j) {
+        if (F[j] == ZERO_EXPONENT) continue;
+        SB += std::ldexp(1.0, 2 * (F[j] - fb));
+        const double b = std::ldexp(l1B[j], -fb);
+        LB += b * b;
+    }
+    bool finite = true;
     for (size_t l = 0; l < ksqA.size(); ++l) {
-        if (!std::isfinite(ksqA[l]) || !std::isfinite(ksqB[l])) return INFINITY;
+        if (!std::isfinite(ksqA[l]) || !std::isfinite(ksqB[l])) finite = false;
         const double a = std::ldexp(ksqA[l], -2 * ea), b = std::ldexp(ksqB[l], -2 * fb);
         NA += a;
         NB += b;
         D += a * b;
     }
-    if (!(D > 0.0)) return INFINITY;
-    return std::sqrt((SA * NB + NA * SB) / (12.0 * D));
+    // the 1-norms are FP64 sums of |x|: a relative margin covers their rounding
+    const double margin = 1.0 + 1e-12;
+    em.sA = std::sqrt(SA);
+    em.sB = std::sqrt(SB);
+    em.a1 = std::sqrt(LA) * margin;
+    em.b1 = std::sqrt(LB) * margin;
+    em.fA = std::sqrt(NA);
+    em.fB = std::sqrt(NB);
+    em.d0 = std::sqrt(D);
+    em.gamma = (finite && D > 0.0) ? std::sqrt((SA * NB + NA * SB) / (12.0 * D)) : std::ldexp(1.0, 30);
+    if (!std::isfinite(em.gamma)) em.gamma = std::ldexp(1.0, 30);
+    return em;
+}
+
+// Rigorous bound on ||X' - X||_F relative to 2^(ea + fb) for P bits (moduli)
+// or s slices.
+double rigorous_num(const ErrModel &em, bool slices, int P_or_s) {
+    if (em.zero) return 0.0;
+    const double kk = (double)em.k;
+    if (!slices) {
+        const double d = std::ldexp(1.0, -P_or_s - 1);
+        return d * (em.sA * em.b1 + em.a1 * em.sB) + kk * d * d * em.sA * em.sB;
+    }
+    const int sl = P_or_s;
+    const double d = std::ldexp(1.0, -7 * sl - 1);
+    double Ds = 0.0;
+    for (int dd = sl + 2; dd <= 2 * sl; ++dd) Ds += (double)(2 * sl + 1 - dd) * 4096.0 * std::ldexp(1.0, -7 * dd);
+    return d * (em.sA * em.b1 + em.a1 * em.sB) + kk * (d * d + Ds) * em.sA * em.sB;
 }
 }  // namespace
 
@@ -455,10 +578,13 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
     if (info) *info = cosma_amd_ozaki_info{};
     if (m == 0 || n == 0) return COSMA_AMD_OK;
     if (k == 0 || alpha == 0.0) return launch_scale(m, n, beta, c, ldc, s);
+    const bool strict = (method & COSMA_AMD_OZAKI_STRICT) != 0;
+    method &= ~COSMA_AMD_OZAKI_STRICT;
     if (tol <= 0.0) tol = 1e-14;
     const int slices = method == COSMA_AMD_OZAKI_SLICES;
     const bool bf16 = method == COSMA_AMD_OZAKI_MODULI_BF16;   // residues as BF16, products on BF16 MFMA
     const int64_t esize = bf16 ? 2 : 1;
+    const bool forced = count > 0;
     if (!slices && count > 0 && (count < 2 || count > MAX_MODULI)) {
         set_error("ozaki: %d moduli requested (2..%d supported)", count, MAX_MODULI);
         return COSMA_AMD_ERR_INVALID_ARG;
@@ -490,21 +616,26 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
     };
     int rc;
 
-    // ---- 1. row / column maxima and sums of squares, per-k sums of squares,
-    // exponents.  aux: maxA [Mp], maxB [Np], ksqA [Kp], ksqB [Kp], rsqA [Mp],
-    // rsqB [Np], flag [1] (cleared together, 8 bytes each), then the
-    // exponents E [Mp] and F [Np]
-    const int64_t nclear = 2 * (Mp + Np) + 2 * Kp + 1;
-    if ((rc = grow(h->ozaki_aux, al((size_t)nclear * 8 + (size_t)(Mp + Np) * 4)))) return rc;
+    // ---- 1. row / column maxima, sums of squares and 1-norms, per-k sums
+    // of squares, exponents.  aux: maxA [Mp], maxB [Np], ksqA [Kp], ksqB [Kp],
+    // rsqA [Mp], rsqB [Np], rabsA [Mp], rabsB [Np], flag [1] (cleared
+    // together, 8 bytes each), then the exponents E [Mp] and F [Np], then
+    // the NORM_SLOTS partial sums of the product's norm
+    const int64_t nclear = 3 * (Mp + Np) + 2 * Kp + 1;
+    const size_t norm_off = (size_t)nclear * 8 + (size_t)(Mp + Np) * 4;
+    if ((rc = grow(h->ozaki_aux, al(norm_off + NORM_SLOTS * 8)))) return rc;
     unsigned long long *maxA = static_cast<unsigned long long *>(h->ozaki_aux.ptr);
     unsigned long long *maxB = maxA + Mp;
     double *ksqA = reinterpret_cast<double *>(maxB + Np);
     double *ksqB = ksqA + Kp;
     double *rsqA = ksqB + Kp;
     double *rsqB = rsqA + Mp;
-    int *flag = reinterpret_cast<int *>(rsqB + Np);
-    int *E = reinterpret_cast<int *>(rsqB + Np + 1);
+    double *rabsA = rsqB + Np;
+    double *rabsB = rabsA + Mp;
+    int *flag = reinterpret_cast<int *>(rabsB + Np);
+    int *E = reinterpret_cast<int *>(rabsB + Np + 1);
     int *F = E + Mp;
+    double *nrm = reinterpret_cast<double *>(static_cast<char *>(h->ozaki_aux.ptr) + norm_off);
     hipLaunchKernelGGL(clear_u64, dim3(grid_for(nclear)), dim3(256), 0, s, maxA, nclear);
     const bool a_rows_contig = ta == 'T', b_rows_contig = tb == 'N';   // op rows contiguous in k
     // grid: x over the contiguous dimension in 64s, y over the other in STATS_SPAN
@@ -512,44 +643,50 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
         const int64_t fast = contig ? K : R, slow = contig ? R : K, span = contig ? 64 * STATS_KC : 64;
         return dim3((unsigned)((fast + span - 1) / span), (unsigned)((slow + STATS_SPAN - 1) / STATS_SPAN));
     };
-    if (a_rows_contig) hipLaunchKernelGGL(stats_kernel<true>, stats_grid(m, k, true), dim3(256), 0, s, a, lda, m, k, maxA, ksqA, rsqA);
-    else hipLaunchKernelGGL(stats_kernel<false>, stats_grid(m, k, false), dim3(256), 0, s, a, lda, m, k, maxA, ksqA, rsqA);
-    if (b_rows_contig) hipLaunchKernelGGL(stats_kernel<true>, stats_grid(n, k, true), dim3(256), 0, s, b, ldb, n, k, maxB, ksqB, rsqB);
-    else hipLaunchKernelGGL(stats_kernel<false>, stats_grid(n, k, false), dim3(256), 0, s, b, ldb, n, k, maxB, ksqB, rsqB);
+    if (a_rows_contig) hipLaunchKernelGGL(stats_kernel<true>, stats_grid(m, k, true), dim3(256), 0, s, a, lda, m, k, maxA, ksqA, rsqA, rabsA);
+    else hipLaunchKernelGGL(stats_kernel<false>, stats_grid(m, k, false), dim3(256), 0, s, a, lda, m, k, maxA, ksqA, rsqA, rabsA);
+    if (b_rows_contig) hipLaunchKernelGGL(stats_kernel<true>, stats_grid(n, k, true), dim3(256), 0, s, b, ldb, n, k, maxB, ksqB, rsqB, rabsB);
+    else hipLaunchKernelGGL(stats_kernel<false>, stats_grid(n, k, false), dim3(256), 0, s, b, ldb, n, k, maxB, ksqB, rsqB, rabsB);
     hipLaunchKernelGGL(exponents_kernel, dim3(grid_for(m)), dim3(256), 0, s, maxA, m, slices, E, flag);
     hipLaunchKernelGGL(exponents_kernel, dim3(grid_for(n)), dim3(256), 0, s, maxB, n, slices, F, flag);
     CA_HIP(hipGetLastError());
     // One read-back of the contiguous span ksqA .. F (each small copy costs
     // tens of microseconds of fixed overhead, which several copies paid in
     // turn while the GPU waited).
-    const size_t span = (size_t)(2 * Kp + Mp + Np + 1) * 8 + (size_t)(Mp + Np) * 4;
+    const size_t span = (size_t)(2 * Kp + 2 * (Mp + Np) + 1) * 8 + (size_t)(Mp + Np) * 4;
     std::vector<double> hbuf((span + 7) / 8);
     CA_HIP(hipMemcpyAsync(hbuf.data(), ksqA, span, hipMemcpyDeviceToHost, s));
     CA_HIP(hipStreamSynchronize(s));
     const double *hb = hbuf.data();
     const std::vector<double> hkA(hb, hb + k), hkB(hb + Kp, hb + Kp + k);
     const std::vector<double> hrA(hb + 2 * Kp, hb + 2 * Kp + m), hrB(hb + 2 * Kp + Mp, hb + 2 * Kp + Mp + n);
+    const double *hl = hb + 2 * Kp + Mp + Np;
+    const std::vector<double> hlA(hl, hl + m), hlB(hl + Mp, hl + Mp + n);
     int nonfinite;
-    std::memcpy(&nonfinite, hb + 2 * Kp + Mp + Np, sizeof(int));
-    const int *hEp = reinterpret_cast<const int *>(hb + 2 * Kp + Mp + Np + 1);
+    std::memcpy(&nonfinite, hl + Mp + Np, sizeof(int));
+    const int *hEp = reinterpret_cast<const int *>(hl + Mp + Np + 1);
     const std::vector<int> hE(hEp, hEp + m), hF(hEp + Mp, hEp + Mp + n);
     if (nonfinite) {
         // Inf / NaN cannot be split; the FP64 MFMA DGEMM propagates them as IEEE does.
-        if (info) info->method = -1;
+        if (info) {
+            info->method = -1;
+            info->error_bound = info->error_estimate = NAN;
+            info->tolerance_met = -1;
+        }
         return launch_dgemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s);
     }
 
-    // ---- precision.  Rounding every entry to P bits below its row / column
-    // maximum leaves errors of variance 2^(2(E_i - P)) / 12 (rows of op(A))
-    // and 2^(2(F_j - P)) / 12 (columns of op(B)); with independent signs
-    //   sum_ij E[dC_ij^2] = 2^-2P / 12 (sum_i 2^2E_i ||B||^2 + ||A||^2 sum_j 2^2F_j),
-    //   sum_ij E[C_ij^2]  = sum_l ||op(A)(:, l)||^2 ||op(B)(l, :)||^2,
-    // so the relative Frobenius error is about gamma * 2^-P.  gamma ~ 0.7 for
-    // entries of similar size and grows when entries sit far below their
-    // row / column maxima.  The count is the smallest whose estimate, doubled,
-    // meets the tolerance.
-    double gamma = error_gamma(hE, hF, hkA, hkB);
-    if (!std::isfinite(gamma)) gamma = std::ldexp(1.0, 30);    // no estimate: most precise count
+    // ---- precision.  The a-priori count: the error model's estimate
+    // relative to the size of X that independent signs would give
+    // (gamma * 2^-P); rounding every entry to P bits below its row / column
+    // maximum gives gamma ~ 0.7 for entries of similar size, more when
+    // entries sit far below their maxima.  After the products the norm of
+    // the computed X replaces that assumed size (cancellation can make X far
+    // smaller), and the count is raised and the product recomputed when the
+    // error relative to it misses the tolerance: the estimate by default, the
+    // rigorous bound with COSMA_AMD_OZAKI_STRICT.
+    const ErrModel em = error_model(hE, hF, hkA, hkB, hlA, hlB, k);
+    const double gamma = em.gamma;
     // Norm bounds of the scaled rows / columns for norm_bits: max over the
     // vectors of ||x|| 2^-E, slightly inflated for the rounding of the FP64
     // sums.  Only used when every exponent lies in [-480, 480], where the
@@ -572,129 +709,205 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
         const int pe = std::min(moduli_bits(cnt, k), MAX_P);
         return norm_bits(moduli_half_limit(cnt), hk, sa, sb, pe);
     };
-    int P = 0;
-    // Slices: the s-digit remainders and the dropped products of weight
-    // 2^-7(s+2) and beyond give about gamma * 2 s 2^-7s; measured errors
-    // reached 1.2x that at 8192^3 (entries a = (u - 1/2) e^z), so the
-    // estimate for slices is doubled twice.
+    // The expected relative error (relative to d0): gamma 2^-P for moduli;
+    // for slices the s-digit remainders and the dropped products of weight
+    // 2^-7(s+2) and beyond give about gamma * 2 s 2^-7s, and measured errors
+    // reached 1.2x that at 8192^3 (entries a = (u - 1/2) e^z), so twice that.
+    // The a-priori count makes twice the expected error meet the tolerance;
+    // after the products, 1.5 times the expected error relative to the
+    // computed X' must meet it (the gap between the two factors keeps the
+    // small difference between d0 and ||X'|| of operands without
+    // cancellation from triggering a second pass).
     auto est = [&](int cnt) {
-        return slices ? 4.0 * gamma * 2.0 * cnt * std::ldexp(1.0, -7 * cnt)
-                      : 2.0 * gamma * std::ldexp(1.0, -bits(cnt));
+        return slices ? 2.0 * gamma * 2.0 * cnt * std::ldexp(1.0, -7 * cnt) : gamma * std::ldexp(1.0, -bits(cnt));
     };
-    if (!slices) {
-        if (count <= 0) {
-            count = 2;
-            while (count < MAX_MODULI && !(bits(count) >= 1 && est(count) <= tol)) ++count;
+    constexpr double PRIOR_FACTOR = 2.0, POST_FACTOR = 1.5;
+    const int max_count = slices ? I8_MAX_PAIRS : MAX_MODULI;
+    // relative errors after the products, with nx = ||X'||_F / 2^(ea + fb)
+    auto post_est = [&](int cnt, double nx) { return em.zero ? 0.0 : est(cnt) * em.d0 / nx; };
+    auto post_rig = [&](int cnt, double nx) {
+        if (em.zero) return 0.0;
+        const double r = rigorous_num(em, slices, slices ? cnt : bits(cnt));
+        return nx > r ? r / (nx - r) : INFINITY;
+    };
+    auto meets = [&](int cnt, double nx) {
+        return strict ? post_rig(cnt, nx) <= tol : POST_FACTOR * post_est(cnt, nx) <= tol;
+    };
+    int P = 0;
+    if (!forced) {
+        count = slices ? 1 : 2;
+        if (!slices) {
+            while (count < max_count && !(bits(count) >= 1 && PRIOR_FACTOR * est(count) <= tol)) ++count;
+        } else {
+            while (count < max_count && PRIOR_FACTOR * est(count) > tol) ++count;
         }
-        P = bits(count);
-        if (P < 1) {
-            set_error("ozaki: %d moduli leave no precision for k = %lld", count, (long long)k);
-            return COSMA_AMD_ERR_UNSUPPORTED;
-        }
-    } else {
-        if (count <= 0) {
-            count = 1;
-            while (count < I8_MAX_PAIRS && est(count) > tol) ++count;
-        }
-        // INT32 exactness of one weight: pairs * k * 127^2 < 2^31
-        if ((double)count * (double)k * 16129.0 >= 2147483648.0) {
+    }
+
+    // the INT8 product kernel's duration (for the caller's roofline), timed
+    // by two events around its launch
+    for (auto &ev : h->ozaki_ev)
+        if (!ev) CA_HIP(hipEventCreate(&ev));
+    const int shift = em.zero ? 0 : em.ea + em.fb + (int)std::ceil(std::log2((double)k + 1.0));
+    int64_t products = 0;
+    int passes = 0;
+    double nx = 0.0, product_ms = 0.0;
+    bool met = false;
+    for (;;) {
+        ++passes;
+        if (!slices) {
+            P = bits(count);
+            if (P < 1) {
+                set_error("ozaki: %d moduli leave no precision for k = %lld", count, (long long)k);
+                return COSMA_AMD_ERR_UNSUPPORTED;
+            }
+        } else if ((double)count * (double)k * 16129.0 >= 2147483648.0) {
+            // INT32 exactness of one weight: pairs * k * 127^2 < 2^31
             set_error("ozaki slices: k = %lld too large for exact INT32 accumulation", (long long)k);
             return COSMA_AMD_ERR_UNSUPPORTED;
         }
-    }
 
-    // ---- workspace: slices / residues of A and B, then the INT8 GEMM output
-    // Planes (one per modulus / slice) are PLANE_PAD bytes longer than their
-    // data: at the usual power-of-two sizes the planes would otherwise start
-    // at addresses 2^26 apart, and the split stores and recombination loads,
-    // which touch the same offset of every plane together, would pile onto
-    // the same memory channels.
-    const int64_t a_plane = Mp * Kp * esize + PLANE_PAD, b_plane = Np * Kp * esize + PLANE_PAD;
-    const int64_t r_plane = ldr * n + PLANE_PAD;
-    const size_t a_bytes = (size_t)count * a_plane, b_bytes = (size_t)count * b_plane;
-    const size_t out_bytes = slices ? (size_t)ldr * n * 8 : (size_t)count * r_plane;
-    if ((rc = grow(h->ozaki_ws, al(a_bytes) + al(b_bytes) + al(out_bytes)))) return rc;
-    char *ws = static_cast<char *>(h->ozaki_ws.ptr);
-    int8_t *As = reinterpret_cast<int8_t *>(ws);
-    int8_t *Bs = reinterpret_cast<int8_t *>(ws + al(a_bytes));
-    void *out = ws + al(a_bytes) + al(b_bytes);
+        // ---- workspace: slices / residues of A and B, then the INT8 GEMM output
+        // Planes (one per modulus / slice) are PLANE_PAD bytes longer than their
+        // data: at the usual power-of-two sizes the planes would otherwise start
+        // at addresses 2^26 apart, and the split stores and recombination loads,
+        // which touch the same offset of every plane together, would pile onto
+        // the same memory channels.
+        const int64_t a_plane = Mp * Kp * esize + PLANE_PAD, b_plane = Np * Kp * esize + PLANE_PAD;
+        const int64_t r_plane = ldr * n + PLANE_PAD;
+        const size_t a_bytes = (size_t)count * a_plane, b_bytes = (size_t)count * b_plane;
+        const size_t out_bytes = slices ? (size_t)ldr * n * 8 : (size_t)count * r_plane;
+        if ((rc = grow(h->ozaki_ws, al(a_bytes) + al(b_bytes) + al(out_bytes)))) return rc;
+        char *ws = static_cast<char *>(h->ozaki_ws.ptr);
+        int8_t *As = reinterpret_cast<int8_t *>(ws);
+        int8_t *Bs = reinterpret_cast<int8_t *>(ws + al(a_bytes));
+        void *out = ws + al(a_bytes) + al(b_bytes);
 
-    // ---- 2. split
-    auto split = [&](const double *x, int64_t ld, int64_t R, int64_t Rpad, bool contig, const int *ex,
-                     int8_t *dst, int64_t stride) {
-        SplitArgs sa{};
-        sa.x = x; sa.ld = ld; sa.R = R; sa.K = k; sa.Kpad = Kp; sa.row_bytes = Kp * esize; sa.slice_stride = stride;
-        sa.E = ex; sa.method = slices ? COSMA_AMD_OZAKI_SLICES : method; sa.esize = (int)esize;
-        sa.count = count; sa.P = P; sa.out = dst;
-        for (int l = 0; l < MAX_MODULI; ++l) { sa.m[l] = kModuli[l]; sa.inv_m[l] = 1.0 / kModuli[l]; }
-        if (contig)
-            hipLaunchKernelGGL(split_contig, dim3(grid_for(Rpad * (Kp / 16))), dim3(256), 0, s, sa, Rpad);
-        else
-            hipLaunchKernelGGL(split_strided, dim3((unsigned)(Rpad / 64), (unsigned)(Kp / 64)), dim3(256), 0, s, sa);
-    };
-    split(a, lda, m, Mp, a_rows_contig, E, As, a_plane);
-    split(b, ldb, n, Np, b_rows_contig, F, Bs, b_plane);
-    CA_HIP(hipGetLastError());
+        // ---- 2. split
+        auto split = [&](const double *x, int64_t ld, int64_t R, int64_t Rpad, bool contig, const int *ex,
+                         int8_t *dst, int64_t stride) {
+            SplitArgs sa{};
+            sa.x = x; sa.ld = ld; sa.R = R; sa.K = k; sa.Kpad = Kp; sa.row_bytes = Kp * esize; sa.slice_stride = stride;
+            sa.E = ex; sa.method = slices ? COSMA_AMD_OZAKI_SLICES : method; sa.esize = (int)esize;
+            sa.count = count; sa.P = P; sa.out = dst;
+            for (int l = 0; l < MAX_MODULI; ++l) { sa.m[l] = kModuli[l]; sa.inv_m[l] = 1.0 / kModuli[l]; }
+            if (contig)
+                hipLaunchKernelGGL(split_contig, dim3(grid_for(Rpad * (Kp / 16))), dim3(256), 0, s, sa, Rpad);
+            else
+                hipLaunchKernelGGL(split_strided, dim3((unsigned)(Rpad / 64), (unsigned)(Kp / 64)), dim3(256), 0, s, sa);
+        };
+        split(a, lda, m, Mp, a_rows_contig, E, As, a_plane);
+        split(b, ldb, n, Np, b_rows_contig, F, Bs, b_plane);
+        CA_HIP(hipGetLastError());
 
-    // ---- 3. INT8 products
-    I8Problem pr{};
-    pr.M = m; pr.N = n; pr.K = Kp * esize;
-    pr.a = As; pr.lda = Kp * esize; pr.a_stride = a_plane;
-    pr.b = Bs; pr.ldb = Kp * esize; pr.b_stride = b_plane;
-    pr.bf16 = bf16;
-    pr.out = out; pr.ldo = ldr;
-    I8Jobs jobs{};
-    int64_t products = 0;
-    if (!slices) {
-        pr.out_stride = r_plane;
-        for (int l = 0; l < count; ++l) {
-            jobs.npairs[l] = 1;
-            jobs.pa[l][0] = (unsigned char)l;
-            jobs.pb[l][0] = (unsigned char)l;
-            jobs.modulus[l] = kModuli[l];
-        }
-        if ((rc = launch_i8gemm(pr, jobs, count, EPI_RES8, s))) return rc;
-        products = count;
+        // ---- 3. INT8 products
+        I8Problem pr{};
+        pr.M = m; pr.N = n; pr.K = Kp * esize;
+        pr.a = As; pr.lda = Kp * esize; pr.a_stride = a_plane;
+        pr.b = Bs; pr.ldb = Kp * esize; pr.b_stride = b_plane;
+        pr.bf16 = bf16;
+        pr.out = out; pr.ldo = ldr;
+        // ---- 4. recombination with the norm of X (beta == 0: C written in
+        // the same pass; otherwise C stays untouched until the check)
+        hipLaunchKernelGGL(clear_u64, dim3(1), dim3(NORM_SLOTS), 0, s, reinterpret_cast<unsigned long long *>(nrm),
+                           (int64_t)NORM_SLOTS);
+        const int mode = beta == 0.0 ? EPI_WRITE_NORM : EPI_NORM;
         CrtConst cc;
-        crt_constants(count, cc);
-        const dim3 gcrt((unsigned)((m + 256 * CRT_ROWS - 1) / (256 * CRT_ROWS)),
-                        (unsigned)std::min<int64_t>(n, 65535));
-        if (crt_limbs(count) == 3)
-            hipLaunchKernelGGL(crt_epilogue<3>, gcrt, dim3(256), 0, s, cc, static_cast<const int8_t *>(out), ldr,
-                               r_plane, m, n, P, E, F, alpha, beta, c, ldc);
-        else
-            hipLaunchKernelGGL(crt_epilogue<NLIMB>, gcrt, dim3(256), 0, s, cc, static_cast<const int8_t *>(out),
-                               ldr, r_plane, m, n, P, E, F, alpha, beta, c, ldc);
-    } else {
-        pr.out_stride = 0;
-        // weight d = t + u (slice indices from 1): pairs with t + u = d, d <= count + 1
-        for (int d = 2; d <= count + 1; ++d) {
-            I8Jobs jd{};
-            int np = 0;
-            for (int t = 1; t <= count; ++t) {
-                const int u = d - t;
-                if (u < 1 || u > count) continue;
-                jd.pa[0][np] = (unsigned char)(t - 1);
-                jd.pb[0][np] = (unsigned char)(u - 1);
-                ++np;
+        const dim3 gcrt((unsigned)((m + 256 * CRT_ROWS - 1) / (256 * CRT_ROWS)), (unsigned)std::min<int64_t>(n, 65535));
+        auto recombine = [&](int md) {
+            const int8_t *res = static_cast<const int8_t *>(out);
+            if (crt_limbs(count) == 3) {
+                if (md == EPI_WRITE) hipLaunchKernelGGL((crt_epilogue<3, EPI_WRITE>), gcrt, dim3(256), 0, s, cc, res, ldr, r_plane, m, n, P, E, F, alpha, beta, c, ldc, shift, nrm);
+                else if (md == EPI_WRITE_NORM) hipLaunchKernelGGL((crt_epilogue<3, EPI_WRITE_NORM>), gcrt, dim3(256), 0, s, cc, res, ldr, r_plane, m, n, P, E, F, alpha, beta, c, ldc, shift, nrm);
+                else hipLaunchKernelGGL((crt_epilogue<3, EPI_NORM>), gcrt, dim3(256), 0, s, cc, res, ldr, r_plane, m, n, P, E, F, alpha, beta, c, ldc, shift, nrm);
+            } else {
+                if (md == EPI_WRITE) hipLaunchKernelGGL((crt_epilogue<NLIMB, EPI_WRITE>), gcrt, dim3(256), 0, s, cc, res, ldr, r_plane, m, n, P, E, F, alpha, beta, c, ldc, shift, nrm);
+                else if (md == EPI_WRITE_NORM) hipLaunchKernelGGL((crt_epilogue<NLIMB, EPI_WRITE_NORM>), gcrt, dim3(256), 0, s, cc, res, ldr, r_plane, m, n, P, E, F, alpha, beta, c, ldc, shift, nrm);
+                else hipLaunchKernelGGL((crt_epilogue<NLIMB, EPI_NORM>), gcrt, dim3(256), 0, s, cc, res, ldr, r_plane, m, n, P, E, F, alpha, beta, c, ldc, shift, nrm);
             }
-            jd.npairs[0] = np;
-            jd.scale[0] = std::ldexp(1.0, -7 * d);
-            pr.accumulate = d > 2;
-            if ((rc = launch_i8gemm(pr, jd, 1, EPI_F64ACC, s))) return rc;
-            products += np;
+        };
+        const dim3 gsl(grid_for(m * n));
+        auto slices_out = [&](int md) {
+            const double *acc = static_cast<const double *>(out);
+            if (md == EPI_WRITE) hipLaunchKernelGGL((slices_epilogue<EPI_WRITE>), gsl, dim3(256), 0, s, acc, ldr, m, n, E, F, alpha, beta, c, ldc, shift, nrm);
+            else if (md == EPI_WRITE_NORM) hipLaunchKernelGGL((slices_epilogue<EPI_WRITE_NORM>), gsl, dim3(256), 0, s, acc, ldr, m, n, E, F, alpha, beta, c, ldc, shift, nrm);
+            else hipLaunchKernelGGL((slices_epilogue<EPI_NORM>), gsl, dim3(256), 0, s, acc, ldr, m, n, E, F, alpha, beta, c, ldc, shift, nrm);
+        };
+        int64_t pass_products = 0;
+        if (!slices) {
+            pr.out_stride = r_plane;
+            I8Jobs jobs{};
+            for (int l = 0; l < count; ++l) {
+                jobs.npairs[l] = 1;
+                jobs.pa[l][0] = (unsigned char)l;
+                jobs.pb[l][0] = (unsigned char)l;
+                jobs.modulus[l] = kModuli[l];
+            }
+            CA_HIP(hipEventRecord(h->ozaki_ev[0], s));
+            if ((rc = launch_i8gemm(pr, jobs, count, EPI_RES8, s))) return rc;
+            CA_HIP(hipEventRecord(h->ozaki_ev[1], s));
+            pass_products = count;
+            crt_constants(count, cc);
+            recombine(mode);
+        } else {
+            pr.out_stride = 0;
+            CA_HIP(hipEventRecord(h->ozaki_ev[0], s));
+            // weight d = t + u (slice indices from 1): pairs with t + u = d, d <= count + 1
+            for (int d = 2; d <= count + 1; ++d) {
+                I8Jobs jd{};
+                int np = 0;
+                for (int t = 1; t <= count; ++t) {
+                    const int u = d - t;
+                    if (u < 1 || u > count) continue;
+                    jd.pa[0][np] = (unsigned char)(t - 1);
+                    jd.pb[0][np] = (unsigned char)(u - 1);
+                    ++np;
+                }
+                jd.npairs[0] = np;
+                jd.scale[0] = std::ldexp(1.0, -7 * d);
+                pr.accumulate = d > 2;
+                if ((rc = launch_i8gemm(pr, jd, 1, EPI_F64ACC, s))) return rc;
+                pass_products += np;
+            }
+            CA_HIP(hipEventRecord(h->ozaki_ev[1], s));
+            slices_out(mode);
         }
-        hipLaunchKernelGGL(slices_epilogue, dim3(grid_for(m * n)), dim3(256), 0, s,
-                           static_cast<const double *>(out), ldr, m, n, E, F, alpha, beta, c, ldc);
+        CA_HIP(hipGetLastError());
+        products += pass_products;
+        double parts[NORM_SLOTS];
+        CA_HIP(hipMemcpyAsync(parts, nrm, sizeof(parts), hipMemcpyDeviceToHost, s));
+        CA_HIP(hipStreamSynchronize(s));
+        float ms = 0.0f;
+        CA_HIP(hipEventElapsedTime(&ms, h->ozaki_ev[0], h->ozaki_ev[1]));
+        product_ms += ms;
+        double sum = 0.0;
+        for (double v : parts) sum += v;
+        nx = std::sqrt(sum) * std::ldexp(1.0, shift - (em.ea + em.fb));   // ||X'||_F / 2^(ea + fb)
+        met = meets(count, nx);
+        const bool last = met || forced || count >= max_count || passes >= 3;
+        if (last) {
+            if (mode == EPI_NORM) {   // beta != 0: C is written now that the count is final
+                if (!slices) recombine(EPI_WRITE);
+                else slices_out(EPI_WRITE);
+                CA_HIP(hipGetLastError());
+            }
+            break;
+        }
+        // smallest larger count that meets the tolerance relative to the
+        // product just computed (the most precise one if none does)
+        int next = count + 1;
+        while (next < max_count && !meets(next, nx)) ++next;
+        count = next;
     }
-    CA_HIP(hipGetLastError());
     if (info) {
         info->method = slices ? COSMA_AMD_OZAKI_SLICES : method;
         info->count = count;
         info->bits = slices ? 7 * count : P;
         info->int8_products = products;
-        // the (doubled) estimate of the relative Frobenius error, see above
-        info->error_bound = est(count);
+        info->error_bound = post_rig(count, nx);
+        info->error_estimate = post_est(count, nx);
+        info->tolerance_met = met ? 1 : 0;
+        info->passes = passes;
+        info->product_ms = product_ms;
     }
     return COSMA_AMD_OK;
 }

@@ -291,6 +291,10 @@ void release_stream_state(cosma_amd_handle_t h) {
     }
     for (auto e : h->events) hipEventDestroy(e);
     h->events.clear();
+    for (auto &e : h->ozaki_ev) {
+        if (e) hipEventDestroy(e);
+        e = nullptr;
+    }
     h->events_used = 0;
     h->dev_c = nullptr;
 }

@@ -81,19 +81,30 @@ int cosma_amd_dgemm(cosma_amd_handle_t handle, char transa, char transb,
 enum {
     COSMA_AMD_OZAKI_MODULI = 0,   /* integer residues modulo coprime moduli <= 256, CRT recombination */
     COSMA_AMD_OZAKI_SLICES = 1,   /* signed 7-bit slices, triangular slice products */
-    COSMA_AMD_OZAKI_MODULI_BF16 = 2 /* the moduli method with BF16 residues: products on the BF16
-                                       matrix cores (v_mfma_f32_16x16x32_bf16), exact FP32 sums
-                                       reduced modulo each modulus every 896 k */
+    COSMA_AMD_OZAKI_MODULI_BF16 = 2, /* the moduli method with BF16 residues: products on the BF16
+                                        matrix cores (v_mfma_f32_16x16x32_bf16), exact FP32 sums
+                                        reduced modulo each modulus every 896 k */
+    COSMA_AMD_OZAKI_STRICT = 0x100 /* flag OR-ed into the method: the tolerance must hold for the
+                                      rigorous error bound, not only for the estimate */
 };
 
 typedef struct {
     int method;             /* method used; -1 if A or B held Inf/NaN (FP64 MFMA DGEMM used instead) */
-    int count;              /* moduli or slices used */
+    int count;              /* moduli or slices used (after any raise by the check) */
     int bits;               /* bits kept per input entry relative to its row/column maximum (moduli: the
                                largest P for which the entrywise or the row/column-norm bound keeps |A'B'|
-                               below half the moduli product) */
-    int64_t int8_products;  /* INT8 (BF16 for COSMA_AMD_OZAKI_MODULI_BF16) GEMMs of size m x n x k computed */
-    double error_bound;     /* estimated relative Frobenius error (doubled), from the inputs' row statistics */
+                               below half the moduli product; slices: 7 * count) */
+    int64_t int8_products;  /* INT8 (BF16 for COSMA_AMD_OZAKI_MODULI_BF16) GEMMs of size m x n x k computed,
+                               over all passes */
+    double error_bound;     /* rigorous bound on ||X' - X||_F / ||X||_F, X = op(A) op(B) and X' the computed
+                               product (before alpha and beta), from the rounding of every input entry and
+                               the norm of X' (NaN for the FP64 fallback; +inf if X' is too small to bound) */
+    double error_estimate;  /* the same ratio as expected with independent rounding errors */
+    int tolerance_met;      /* 1: the criterion meets the tolerance (1.5 x the estimate, or the bound with
+                               COSMA_AMD_OZAKI_STRICT); 0: it does not even at the count used (the largest
+                               count, or a count given by the caller); -1: FP64 fallback */
+    int passes;             /* product passes computed: 1, or more when the check raised the count */
+    double product_ms;      /* device time of the INT8 / BF16 product launches, all passes */
 } cosma_amd_ozaki_info;
 
 /*
@@ -105,22 +116,37 @@ typedef struct {
  * computed, and FP64 recombines them and applies alpha and beta (beta == 0
  * never reads C).  FP64 arithmetic runs only in the scaling, the
  * recombination and the epilogue.
- * tolerance: requested relative accuracy (<= 0 means 1e-14); the count of
- * moduli or slices is then chosen from the data: the row/column maxima and
- * sums of squares of op(A) and op(B) give an estimate of the relative
- * Frobenius error of each precision, and the smallest count whose doubled
- * estimate meets the tolerance is used.  count > 0 forces the number of
- * moduli (2..20) or slices (1..8) instead.  info (may be NULL) receives the
- * method, count, bits and the doubled error estimate.
+ * tolerance: requested relative accuracy (<= 0 means 1e-14) of the product
+ * X = op(A) op(B) in the Frobenius norm, ||X' - X||_F <= tol ||X||_F.  The
+ * count of moduli or slices is chosen from the data in two steps.  Before the
+ * products, the row/column maxima and sums of squares of op(A) and op(B) give
+ * an estimate of the error of each precision relative to the size X would
+ * have with independent signs, and the smallest count for which twice the
+ * estimate meets the tolerance is used.  After the products, the norm of the
+ * computed X' (summed in the recombination) replaces that assumed size: when
+ * cancellation makes X smaller and the error relative to it misses the
+ * tolerance, the count is raised and the product recomputed (with beta != 0,
+ * C is read and written only once, with the final count).  The check asks
+ * 1.5 times the estimate to meet the tolerance, or with COSMA_AMD_OZAKI_STRICT
+ * OR-ed into `method` the rigorous bound (from the rounding of every entry
+ * and the row/column 1-norms).  count > 0 forces
+ * the number of moduli (2..20) or slices (1..8) instead (no raise).  info
+ * (may be NULL) receives the method, count, bits, the rigorous bound and the
+ * estimate for the result, whether the tolerance was met, and the product
+ * time.  Inputs whose error cannot meet the tolerance at the largest count
+ * (strong cancellation: P is at most 58 bits below each row/column maximum)
+ * are computed at that count and reported with tolerance_met = 0.
  * The BF16 variant (COSMA_AMD_OZAKI_MODULI_BF16) holds the same residues as
  * BF16 numbers (exact: integers in [-128, 127]) and multiplies them on the
  * BF16 matrix cores; it chooses the same count and gives bit-identical
  * results at twice the operand bytes.
- * Limits: moduli need k < 131072 (INT8; the BF16 variant has no such limit)
- * and slices need slices * k * 127^2 < 2^31 (exact INT32 sums);
- * COSMA_AMD_ERR_UNSUPPORTED otherwise.
- * Uses workspaces owned by the handle; asynchronous except for one read-back
- * of the row statistics (non-finite flag, exponents, sums of squares).
+ * Limits: moduli need k < 131072 (INT8 sums exact in INT32), the BF16 variant
+ * needs round_up(k, 256) < 2^22 (32-bit offsets in the product kernel), and
+ * slices need slices * k * 127^2 < 2^31; COSMA_AMD_ERR_UNSUPPORTED otherwise
+ * (checked before any work is queued).
+ * Uses workspaces owned by the handle; returns when C is final (one read-back
+ * of the row statistics before the products, one of the product's norm per
+ * pass).
  * No counterpart in the reference (its local multiply is a host or vendor
  * FP64 GEMM, src/cosma/local_multiply.cpp:299-381); same argument
  * conventions as cosma_amd_dgemm.

@@ -131,6 +131,14 @@ def dgemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, dd=False):
         raise ValueError("oracle_dgemm: invalid arguments")
 
 
+def dgemm_raw(ta, tb, m, n, k, alpha, pa, lda, pb, ldb, beta, pc, ldc, dd=False):
+    """As dgemm, on raw host addresses (for sub-blocks of large host buffers)."""
+    f = lib().oracle_dgemm_dd if dd else lib().oracle_dgemm
+    rc = f(ta.encode(), tb.encode(), m, n, k, alpha, pa, lda, pb, ldb, beta, pc, ldc)
+    if rc:
+        raise ValueError("oracle_dgemm: invalid arguments")
+
+
 def block_transform(op, rows, cols, alpha, src, lds, beta, dst, ldd):
     rc = lib().oracle_block_transform(op.encode(), rows, cols, alpha, _p(src), lds, beta, _p(dst), ldd)
     if rc:

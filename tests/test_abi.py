@@ -78,7 +78,8 @@ def test_header_is_plain_c():
 def test_python_constants_match_the_header():
     # The enum values the Python binding passes through the ABI are the ones
     # the header declares (compiled and printed by the C compiler).
-    names = ["COSMA_AMD_OZAKI_MODULI", "COSMA_AMD_OZAKI_SLICES", "COSMA_AMD_OZAKI_MODULI_BF16"]
+    names = ["COSMA_AMD_OZAKI_MODULI", "COSMA_AMD_OZAKI_SLICES", "COSMA_AMD_OZAKI_MODULI_BF16",
+             "COSMA_AMD_OZAKI_STRICT"]
     src = "#include <stdio.h>\n#include \"cosma_amd.h\"\nint main(void){" + "".join(
         f'printf("%d\\n", (int){n});' for n in names) + "return 0;}\n"
     import tempfile
@@ -88,4 +89,22 @@ def test_python_constants_match_the_header():
                            input=src, capture_output=True, text=True)
         assert r.returncode == 0, r.stderr
         values = [int(v) for v in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
-    assert values == [ca.OZAKI_MODULI, ca.OZAKI_SLICES, ca.OZAKI_MODULI_BF16]
+    assert values == [ca.OZAKI_MODULI, ca.OZAKI_SLICES, ca.OZAKI_MODULI_BF16, ca.OZAKI_STRICT]
+
+
+def test_info_struct_layout_matches_the_header():
+    # size and field offsets of cosma_amd_ozaki_info as the C compiler lays it out
+    import ctypes
+    fields = [f for f, _ in ca.OzakiInfo._fields_]
+    src = "#include <stdio.h>\n#include <stddef.h>\n#include \"cosma_amd.h\"\nint main(void){" + \
+        'printf("%zu\\n", sizeof(cosma_amd_ozaki_info));' + "".join(
+            f'printf("%zu\\n", offsetof(cosma_amd_ozaki_info, {f}));' for f in fields) + "return 0;}\n"
+    import tempfile
+    with tempfile.TemporaryDirectory() as tmp:
+        exe = os.path.join(tmp, "layout")
+        r = subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), "-x", "c", "-", "-o", exe],
+                           input=src, capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+        values = [int(v) for v in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
+    assert values[0] == ctypes.sizeof(ca.OzakiInfo)
+    assert values[1:] == [getattr(ca.OzakiInfo, f).offset for f in fields]

@@ -1,0 +1,264 @@
+"""Correctness at the benchmark sizes, where the oracle cannot form all of C.
+
+Every workload the benchmark times is run here at its full size and checked
+  * for finiteness of every element of C,
+  * against the double-double oracle on whole sampled rows and columns of C
+    (all k terms each; tests/fullsize.py), at the acceptance bound of the
+    path: relative Frobenius error <= 4 k 2^-53 for FP64 results, <= the
+    requested tolerance (1e-14) for split precision,
+  * where it is cheap, against the plain FP64 GPU DGEMM on all of C.
+Workloads: 16384^3 A^T B + 0.5 C from pinned host memory in 4096^2 tiles;
+the tall-skinny m = n = 2048, k = 2^20 A^T B + C streamed in 65536-deep
+k-panels (A and B have 2^31 elements each, beyond 32-bit indexing); the
+8192^3 split-precision DGEMM on a = (u - 1/2) exp(phi z) for phi in {0, 1, 4}
+with INT8 residues, BF16 residues and slices; the 4096-problem log-uniform
+[16, 512] batch in its plain and split-precision grouped forms.
+"""
+import os
+import sys
+import time
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle.sampled import dd_lines, host_threads, lines_error
+from helpers import U
+
+pytestmark = [pytest.mark.gpu, pytest.mark.timeout(900)]
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def say(msg):
+    print(f"[fullsize] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def fill_pinned(t, seed):
+    """Uniform [-1, 1) fill of a pinned host tensor (cols, rows) = column-major
+    rows x cols, generated on the GPU in 2 GiB pieces (same values as
+    cosma_amd_fill_uniform of the whole matrix)."""
+    import torch
+    import cosma_amd as ca
+    cols, rows = t.shape
+    step = max(1, (1 << 28) // rows)
+    tmp = torch.empty((min(step, cols), rows), dtype=torch.float64, device="cuda")
+    for c0 in range(0, cols, step):
+        cc = min(step, cols - c0)
+        ca.fill_uniform(tmp.data_ptr(), rows, cc, rows, seed, offset=c0 * rows)
+        t[c0:c0 + cc].copy_(tmp[:cc])
+    torch.cuda.synchronize()
+    del tmp
+
+
+def streamed_case(m, n, k, alpha, beta, tiles, rows, cols, seeds):
+    """C <- alpha A^T B + beta C with A (k x m), B (k x n), C (m x n) in pinned
+    host memory through the host-streaming DGEMM; returns the errors."""
+    import torch
+    import cosma_amd as ca
+    say(f"streamed A^T B m={m} n={n} k={k} alpha={alpha} beta={beta} tiles={tiles}: allocating pinned operands")
+    A = torch.empty((m, k), dtype=torch.float64, pin_memory=True)     # column i = row i of A^T
+    B = torch.empty((n, k), dtype=torch.float64, pin_memory=True)
+    C = torch.empty((n, m), dtype=torch.float64, pin_memory=True)
+    for t, s in zip((A, B, C), seeds):
+        fill_pinned(t, s)
+    C0_rows = {i: C[:, i].numpy().copy() for i in rows}
+    C0_cols = {j: C[j, :].numpy().copy() for j in cols}
+    h = ca.Handle(0, streams=2, tile_m=tiles[0], tile_n=tiles[1], tile_k=tiles[2])
+    say("streaming call")
+    t0 = time.perf_counter()
+    h.dgemm_host("T", "N", m, n, k, alpha, A.data_ptr(), k, B.data_ptr(), k, beta, C.data_ptr(), m)
+    say(f"streaming call done in {time.perf_counter() - t0:.3f} s, schedule {h.last_mode()}")
+    assert h.last_mode() == 1
+    finite = bool(torch.isfinite(C).all())
+    # the same product by the device-resident FP64 DGEMM, all of C
+    say("device-resident reference product")
+    dA, dB = A.cuda(), B.cuda()
+    dC = torch.empty((n, m), dtype=torch.float64, device="cuda")
+    ca.fill_uniform(dC.data_ptr(), m, n, m, seeds[2])
+    h.dgemm("T", "N", m, n, k, alpha, dA.data_ptr(), k, dB.data_ptr(), k, beta, dC.data_ptr(), m)
+    torch.cuda.synchronize()
+    diff = float(torch.linalg.vector_norm(dC.cpu() - C)) / float(torch.linalg.vector_norm(dC))
+    del dA, dB, dC
+    torch.cuda.empty_cache()
+    say(f"double-double oracle on rows {rows} and columns {cols} ({host_threads()} threads)")
+    t0 = time.perf_counter()
+    r, c = dd_lines("T", "N", m, n, k, A.data_ptr(), k, B.data_ptr(), k, rows, cols)
+    say(f"oracle done in {time.perf_counter() - t0:.1f} s")
+    ref_r = {i: alpha * r[i] + beta * C0_rows[i] for i in rows}
+    ref_c = {j: alpha * c[j] + beta * C0_cols[j] for j in cols}
+    got_r = {i: C[:, i].numpy() for i in rows}
+    got_c = {j: C[j, :].numpy() for j in cols}
+    err = lines_error(got_r, got_c, ref_r, ref_c)
+    h.close()
+    return finite, diff, err
+
+
+def test_host_resident_16384_tiles_4096():
+    m = n = k = 16384
+    finite, diff, err = streamed_case(m, n, k, 1.0, 0.5, (4096, 4096, 4096), rows=(0, 4097, 16383),
+                                      cols=(0, 9000, 16383), seeds=(11, 12, 13))
+    say(f"host 16384^3: finite={finite} vs device DGEMM {diff:.3e} vs double-double {err:.3e}")
+    assert finite
+    assert err <= 4 * k * U
+    assert diff <= 8 * k * U
+
+
+def test_tall_skinny_k_2pow20_panels_65536():
+    """A and B hold 2048 * 2^20 = 2^31 elements each: offsets past 2^31
+    elements (and 2^34 bytes) must be formed in 64 bits."""
+    m = n = 2048
+    k = 1 << 20
+    finite, diff, err = streamed_case(m, n, k, 1.0, 1.0, (4096, 4096, 65536), rows=(0, 1023, 2047),
+                                      cols=(0, 2047), seeds=(21, 22, 23))
+    say(f"tall-skinny: finite={finite} vs device DGEMM {diff:.3e} vs double-double {err:.3e}")
+    assert finite
+    assert err <= 4 * k * U
+    assert diff <= 8 * k * U
+
+
+_OZ_REF = {}
+
+
+def _ozaki_operands(phi, n=8192):
+    import torch
+    import cosma_amd as ca
+    A = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    B = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    ca.fill_lognormal(A.data_ptr(), n, n, n, 21, phi)
+    ca.fill_lognormal(B.data_ptr(), n, n, n, 22, phi)
+    torch.cuda.synchronize()
+    if phi not in _OZ_REF:
+        say(f"double-double oracle for the 8192^3 product at phi={phi}")
+        Ah, Bh = A.cpu(), B.cpu()
+        rows, cols = (0, 4097, 8191), (1, 6000)
+        r, c = dd_lines("N", "N", n, n, n, Ah.data_ptr(), n, Bh.data_ptr(), n, rows, cols)
+        _OZ_REF[phi] = (rows, cols, r, c)
+    return A, B
+
+
+@pytest.mark.parametrize("phi", [0.0, 1.0, 4.0])
+@pytest.mark.parametrize("method", ["moduli", "moduli_bf16", "slices"])
+def test_split_precision_8192(phi, method):
+    import torch
+    import cosma_amd as ca
+    n = 8192
+    code = {"moduli": ca.OZAKI_MODULI, "moduli_bf16": ca.OZAKI_MODULI_BF16, "slices": ca.OZAKI_SLICES}[method]
+    A, B = _ozaki_operands(phi)
+    C = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    h = ca.Handle(0)
+    say(f"split precision 8192^3 NN phi={phi} method={method} tolerance=1e-14")
+    info = h.dgemm_ozaki("N", "N", n, n, n, 1.0, A.data_ptr(), n, B.data_ptr(), n, 0.0, C.data_ptr(), n,
+                         method=code, tolerance=1e-14)
+    torch.cuda.synchronize()
+    rows, cols, r, c = _OZ_REF[phi]
+    Ch = C.cpu().numpy()             # (col, row)
+    err = lines_error({i: Ch[:, i] for i in rows}, {j: Ch[j, :] for j in cols}, r, c)
+    finite = bool(torch.isfinite(C).all())
+    say(f"phi={phi} {method}: count={info['count']} bits={info['bits']} estimate={info['error_bound']:.2e} "
+        f"error={err:.2e}")
+    h.close()
+    assert finite
+    assert err <= 1e-14
+    assert info["method"] == code
+
+
+def _batch():
+    sys.path.insert(0, ROOT)
+    import bench
+    return bench.batch_dims()
+
+
+@pytest.fixture(scope="module")
+def batch():
+    """The benchmark's 4096 NN problems packed in three device buffers."""
+    import torch
+    import cosma_amd as ca
+    dims = _batch()
+    m, n, k = dims[:, 0], dims[:, 1], dims[:, 2]
+
+    def offsets(sizes):
+        sizes = (sizes + 1) // 2 * 2
+        return np.concatenate([[0], np.cumsum(sizes)])
+    oa, ob, oc = offsets(m * k), offsets(k * n), offsets(m * n)
+    A = torch.empty(int(oa[-1]), dtype=torch.float64, device="cuda")
+    B = torch.empty(int(ob[-1]), dtype=torch.float64, device="cuda")
+    C = torch.empty(int(oc[-1]), dtype=torch.float64, device="cuda")
+    ca.fill_uniform(A.data_ptr(), int(oa[-1]), 1, int(oa[-1]), 41)
+    ca.fill_uniform(B.data_ptr(), int(ob[-1]), 1, int(ob[-1]), 42)
+    torch.cuda.synchronize()
+    pa, pb, pc = A.data_ptr(), B.data_ptr(), C.data_ptr()
+    probs = [("N", "N", int(m[i]), int(n[i]), int(k[i]), 1.0, pa + 8 * int(oa[i]), int(m[i]), pb + 8 * int(ob[i]),
+              int(k[i]), 0.0, pc + 8 * int(oc[i]), int(m[i])) for i in range(len(dims))]
+    # sampled problems: the 12 largest, the 12 smallest (by flops) and 24 spread over the batch
+    order = np.argsort(m * n * k)
+    sample = sorted(set(order[:12].tolist() + order[-12:].tolist() + list(range(0, len(dims), len(dims) // 24))))
+    Ah, Bh = A.cpu().numpy(), B.cpu().numpy()
+    say(f"double-double oracle for {len(sample)} sampled problems of the batch")
+
+    def ref(i):
+        out = np.zeros((int(m[i]), int(n[i])), order="F")
+        oracle.dgemm_raw("N", "N", int(m[i]), int(n[i]), int(k[i]), 1.0, Ah.ctypes.data + 8 * int(oa[i]), int(m[i]),
+                         Bh.ctypes.data + 8 * int(ob[i]), int(k[i]), 0.0, out.ctypes.data, int(m[i]), dd=True)
+        return out
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(host_threads()) as ex:
+        refs = dict(zip(sample, ex.map(ref, sample)))
+    return {"dims": dims, "oc": oc, "A": A, "B": B, "C": C, "probs": probs, "refs": refs}
+
+
+def _batch_errors(batch):
+    import torch
+    C = batch["C"].cpu().numpy()
+    dims, oc = batch["dims"], batch["oc"]
+    finite = bool(torch.isfinite(batch["C"]).all())
+    errs = {}
+    for i, ref in batch["refs"].items():
+        mi, ni = int(dims[i, 0]), int(dims[i, 1])
+        got = C[int(oc[i]):int(oc[i]) + mi * ni].reshape((ni, mi)).T
+        errs[i] = oracle.rel_fro(got, ref)
+    return finite, errs
+
+
+def test_batch_4096_plain_grouped(batch):
+    import torch
+    import cosma_amd as ca
+    h = ca.Handle(0)
+    say("grouped FP64 plan over the 4096-problem batch: tune, graph replay")
+    plan = ca.Grouped(h, batch["probs"])
+    plan.tune()
+    batch["C"].fill_(float("nan"))
+    plan.graph_run()
+    plan.graph_run()
+    torch.cuda.synchronize()
+    finite, errs = _batch_errors(batch)
+    worst = max(errs, key=errs.get)
+    say(f"plain batch: finite={finite}, worst sampled error {errs[worst]:.2e} (problem {worst}, "
+        f"dims {batch['dims'][worst].tolist()}), plan {plan.info()}")
+    plan.close()
+    h.close()
+    assert finite
+    for i, e in errs.items():
+        assert e <= 4 * int(batch["dims"][i, 2]) * U, (i, e)
+
+
+def test_batch_4096_split_precision_grouped(batch):
+    import torch
+    import cosma_amd as ca
+    h = ca.Handle(0)
+    say("grouped split-precision plan over the 4096-problem batch at 1e-14: tune, graph replay")
+    plan = ca.GroupedOzaki(h, batch["probs"], tolerance=1e-14)
+    plan.tune()
+    batch["C"].fill_(float("nan"))
+    plan.graph_run()
+    plan.graph_run()
+    torch.cuda.synchronize()
+    finite, errs = _batch_errors(batch)
+    worst = max(errs, key=errs.get)
+    say(f"split batch: finite={finite}, worst sampled error {errs[worst]:.2e} (problem {worst}), "
+        f"plan {plan.info()}")
+    plan.close()
+    h.close()
+    assert finite
+    assert max(errs.values()) <= 1e-14

@@ -255,3 +255,23 @@ def test_host_blas_correct_with_torch_loaded():
     ref = np.zeros((m, n), order="F")
     oracle.dgemm("N", "N", m, n, k, 1.0, A, m, B, k, 0.0, ref, m)
     assert oracle.rel_fro(C, ref) <= gemm_bound(k)
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T"), ("T", "T")])
+def test_sampled_lines_helper_matches_full_product(ta, tb):
+    """The full-size checks take whole rows / columns of op(A) op(B) from the
+    oracle through sub-block addresses; they must equal the same lines of the
+    oracle's full product."""
+    from oracle.sampled import dd_lines, lines_error
+    m, n, k = 37, 29, 45
+    A = oracle.uniform(*((m, k) if ta == "N" else (k, m)), 71)
+    B = oracle.uniform(*((k, n) if tb == "N" else (n, k)), 72)
+    full = np.zeros((m, n), order="F")
+    oracle.dgemm(ta, tb, m, n, k, 1.0, A, A.shape[0], B, B.shape[0], 0.0, full, m, dd=True)
+    rows, cols = (0, 17, 36), (0, 28)
+    r, c = dd_lines(ta, tb, m, n, k, A.ctypes.data, A.shape[0], B.ctypes.data, B.shape[0], rows, cols, threads=3)
+    for i in rows:
+        assert np.array_equal(r[i], full[i, :])
+    for j in cols:
+        assert np.array_equal(c[j], full[:, j])
+    assert lines_error(r, c, {i: full[i, :] for i in rows}, {j: full[:, j] for j in cols}) == 0.0

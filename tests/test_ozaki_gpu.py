@@ -170,6 +170,7 @@ def test_default_tolerance_against_double_double(handle, method, m, n, k, ta, tb
     err = oracle.rel_fro(got, ref)
     assert err <= 1e-14, (err, info)
     assert err <= info["error_bound"], (err, info)
+    assert info["tolerance_met"] == 1 and info["passes"] == 1, info
 
 
 @pytest.mark.parametrize("count", [2, 9, 14, 20])

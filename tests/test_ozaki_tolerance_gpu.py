@@ -1,0 +1,166 @@
+"""The split-precision tolerance holds for the product the caller gets,
+also when the operands cancel or their rounding errors are correlated.
+
+The requested tolerance bounds ||X' - X||_F / ||X||_F for X = op(A) op(B).
+The a-priori count assumes the size X would have with independent signs;
+the norm of the computed X' then decides whether the count was enough, and
+the product is recomputed with more moduli (or slices) when it was not.
+Cases:
+  * cancellation: A = [Y, Y], B = [Z; -Z + eps W], so X = eps Y W is far
+    smaller than |A||B| (the error of the rounded inputs does not shrink
+    with it);
+  * correlated rounding errors: every entry of A has the same value, so all
+    of its rounding errors are equal, and B >= 0 adds them coherently;
+  * strong cancellation beyond what 58 bits per entry can resolve: the call
+    reports that the tolerance is not met, with an error bound that holds;
+  * the rigorous bound (COSMA_AMD_OZAKI_STRICT) and forced counts.
+Every result is compared with the double-double oracle; the rigorous bound
+returned in the info must hold for every case.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import from_dev, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def run(handle, A, B, method, tol=1e-14, count=0, beta=0.0, C0=None):
+    import torch
+    m, k = A.shape
+    n = B.shape[1]
+    C0 = np.zeros((m, n), order="F") if C0 is None else C0
+    dA, lda = to_dev(A)
+    dB, ldb = to_dev(B)
+    dC, ldc = to_dev(C0)
+    info = handle.dgemm_ozaki("N", "N", m, n, k, 1.0, dA.data_ptr(), lda, dB.data_ptr(), ldb, beta,
+                              dC.data_ptr(), ldc, method=method, tolerance=tol, count=count)
+    torch.cuda.synchronize()
+    got = from_dev(dC, m)
+    ref = C0.copy(order="F")
+    oracle.dgemm("N", "N", m, n, k, 1.0, A, m, B, k, beta, ref, m, dd=True)
+    X = np.zeros((m, n), order="F")
+    oracle.dgemm("N", "N", m, n, k, 1.0, A, m, B, k, 0.0, X, m, dd=True)
+    # error of the product X' (what the tolerance is about), from C = X' + beta C0
+    err = np.linalg.norm((got - ref).ravel()) / np.linalg.norm(X.ravel())
+    return got, ref, err, info
+
+
+def cancelling(m, n, half, eps, seed):
+    Y = oracle.uniform(m, half, seed)
+    Z = oracle.uniform(half, n, seed + 1)
+    W = oracle.uniform(half, n, seed + 2)
+    A = np.asfortranarray(np.hstack([Y, Y]))
+    B = np.asfortranarray(np.vstack([Z, -Z + eps * W]))
+    return A, B
+
+
+def methods():
+    import cosma_amd as ca
+    return {"moduli": ca.OZAKI_MODULI, "moduli_bf16": ca.OZAKI_MODULI_BF16, "slices": ca.OZAKI_SLICES}
+
+
+@pytest.mark.parametrize("name", ["moduli", "moduli_bf16", "slices"])
+@pytest.mark.parametrize("eps", [2.0 ** -6, 2.0 ** -9])
+def test_cancellation_meets_tolerance(handle, name, eps):
+    method = methods()[name]
+    A, B = cancelling(96, 80, 512, eps, 31)
+    got, ref, err, info = run(handle, A, B, method)
+    X_norm_ratio = eps  # ||X|| is about eps times what independent signs would give
+    assert err <= info["error_bound"], (err, info)
+    if info["tolerance_met"] == 1:
+        assert err <= 1e-14, (err, info)
+        assert 1.5 * info["error_estimate"] <= 1e-14
+    else:
+        # only the slices (56 bits at most) may fall short here
+        assert name == "slices" and info["count"] == 8, info
+    if name != "slices":
+        assert info["tolerance_met"] == 1, info
+        # the a-priori count (sized for |X| ~ |A||B|) was not enough: the check raised it
+        assert info["passes"] >= 2 and info["int8_products"] > info["count"], info
+    assert X_norm_ratio < 1e-2
+
+
+@pytest.mark.parametrize("name", ["moduli", "slices"])
+def test_cancellation_with_beta_reads_c_once(handle, name):
+    """beta != 0: C is read and written once, after the count is final."""
+    method = methods()[name]
+    A, B = cancelling(64, 72, 256, 2.0 ** -10, 41)
+    C0 = oracle.uniform(64, 72, 43)
+    got, ref, err, info = run(handle, A, B, method, beta=0.75, C0=C0)
+    assert info["passes"] >= 2
+    assert err <= info["error_bound"]
+    # the result holds beta * C0 once (not beta^2 C0 or X' twice)
+    assert np.linalg.norm((got - ref).ravel()) <= 1e-12 * np.linalg.norm(ref.ravel()), (err, info)
+
+
+def test_correlated_rounding_errors(handle):
+    """All entries of A equal: P-bit rounding moves each by the same amount,
+    and with B >= 0 the errors add up coherently over k (the independence
+    behind the estimate does not hold).  The estimate-based default must still
+    meet the tolerance on the result, and the strict mode must meet it with
+    its rigorous bound."""
+    import cosma_amd as ca
+    m, n, k = 64, 64, 2048
+    A = np.full((m, k), 1.0 / 3.0, order="F")
+    B = np.asfortranarray(0.5 * (oracle.uniform(k, n, 51) + 1.0))
+    for method in (ca.OZAKI_MODULI, ca.OZAKI_SLICES):
+        got, ref, err, info = run(handle, A, B, method)
+        assert err <= 1e-14 and err <= info["error_bound"], (method, err, info)
+        got, ref, err, info = run(handle, A, B, method | ca.OZAKI_STRICT)
+        assert info["tolerance_met"] == 1, info
+        assert err <= info["error_bound"] <= 1e-14, (method, err, info)
+
+
+def test_strict_mode_on_cancellation(handle):
+    import cosma_amd as ca
+    A, B = cancelling(96, 80, 512, 2.0 ** -8, 61)
+    _, _, err_d, info_d = run(handle, A, B, ca.OZAKI_MODULI)
+    _, _, err_s, info_s = run(handle, A, B, ca.OZAKI_MODULI | ca.OZAKI_STRICT)
+    assert info_s["tolerance_met"] == 1 and info_s["error_bound"] <= 1e-14, info_s
+    assert err_s <= info_s["error_bound"]
+    assert info_s["count"] >= info_d["count"]
+    assert err_d <= info_d["error_bound"]
+
+
+def test_unreachable_tolerance_is_reported(handle):
+    """X = 2^-45 Y W: resolving it to 1e-14 needs more than 58 bits per entry.
+    The call uses the largest count, says the tolerance is not met, and its
+    bound still holds."""
+    import cosma_amd as ca
+    A, B = cancelling(64, 64, 256, 2.0 ** -45, 71)
+    got, ref, err, info = run(handle, A, B, ca.OZAKI_MODULI)
+    assert info["tolerance_met"] == 0 and info["count"] == 20, info
+    assert err <= info["error_bound"], (err, info)
+    assert 1.5 * info["error_estimate"] > 1e-14
+
+
+def test_forced_count_is_not_raised(handle):
+    import cosma_amd as ca
+    A, B = cancelling(64, 64, 256, 2.0 ** -10, 81)
+    _, _, err, info = run(handle, A, B, ca.OZAKI_MODULI, count=12)
+    assert info["count"] == 12 and info["passes"] == 1 and info["int8_products"] == 12
+    assert info["tolerance_met"] == 0 and err <= info["error_bound"]
+    _, _, err, info = run(handle, oracle.uniform(64, 256, 1), oracle.uniform(256, 64, 2), ca.OZAKI_MODULI, count=16)
+    assert info["tolerance_met"] == 1 and info["passes"] == 1 and err <= info["error_bound"]
+
+
+def test_uniform_inputs_take_one_pass(handle):
+    """Without cancellation the a-priori count is kept: one pass, and the
+    bound and estimate after the products are consistent with the error."""
+    import cosma_amd as ca
+    for method in (ca.OZAKI_MODULI, ca.OZAKI_MODULI_BF16, ca.OZAKI_SLICES):
+        _, _, err, info = run(handle, oracle.uniform(128, 512, 3), oracle.uniform(512, 96, 4), method)
+        assert info["passes"] == 1 and info["tolerance_met"] == 1, info
+        assert err <= 2.0 * info["error_estimate"] and 1.5 * info["error_estimate"] <= 1e-14, (err, info)
+        assert err <= info["error_bound"]
+        assert info["product_ms"] > 0.0
+
+
+def test_zero_operand(handle):
+    import cosma_amd as ca
+    A = np.zeros((32, 40), order="F")
+    got, _, _, info = run(handle, A, oracle.uniform(40, 24, 5), ca.OZAKI_MODULI)
+    assert np.array_equal(got, np.zeros((32, 24)))
+    assert info["tolerance_met"] == 1 and info["error_bound"] == 0.0
