@@ -78,6 +78,9 @@ SIGNATURES = {
     "cosma_amd_version": (ctypes.c_char_p, []),
     "cosma_amd_handle_create": (ctypes.c_int, [ctypes.POINTER(c_vp), ctypes.c_int, ctypes.c_int, c_i64, c_i64, c_i64]),
     "cosma_amd_handle_destroy": (ctypes.c_int, [c_vp]),
+    "cosma_amd_handle_get_config": (ctypes.c_int, [c_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                                   ctypes.POINTER(c_i64), ctypes.POINTER(c_i64),
+                                                   ctypes.POINTER(c_i64)]),
     "cosma_amd_dgemm": (ctypes.c_int, [c_vp, c_char, c_char, c_i64, c_i64, c_i64, c_dbl,
                                        c_vp, c_i64, c_vp, c_i64, c_dbl, c_vp, c_i64, c_vp]),
     "cosma_amd_fill_uniform": (ctypes.c_int, [c_vp, c_i64, c_i64, c_i64, ctypes.c_uint64, ctypes.c_uint64, c_vp]),
@@ -108,6 +111,8 @@ SIGNATURES = {
     "cosma_amd_grouped_ozaki_set_grid": (ctypes.c_int, [c_vp, ctypes.c_int]),
     "cosma_amd_grouped_ozaki_info": (ctypes.c_int, [c_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),
                                                     ctypes.POINTER(c_i64), ctypes.POINTER(ctypes.c_int)]),
+    "cosma_amd_grouped_ozaki_check": (ctypes.c_int, [c_vp, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
+                                                     ctypes.POINTER(c_i64), ctypes.POINTER(ctypes.c_int), c_vp]),
     "cosma_amd_grouped_ozaki_destroy": (ctypes.c_int, [c_vp]),
     "cosma_amd_host_alloc": (ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(c_vp)]),
     "cosma_amd_host_free": (ctypes.c_int, [c_vp]),
@@ -166,6 +171,13 @@ class Handle:
         h = c_vp()
         check(lib().cosma_amd_handle_create(ctypes.byref(h), device, streams, tile_m, tile_n, tile_k))
         self.ptr = h
+
+    def config(self):
+        """(device, streams, tile_m, tile_n, tile_k) of the handle."""
+        d, st, tm, tn, tk = ctypes.c_int(), ctypes.c_int(), c_i64(), c_i64(), c_i64()
+        check(lib().cosma_amd_handle_get_config(self.ptr, ctypes.byref(d), ctypes.byref(st), ctypes.byref(tm),
+                                                ctypes.byref(tn), ctypes.byref(tk)))
+        return d.value, st.value, tm.value, tn.value, tk.value
 
     def close(self):
         if self.ptr:
@@ -306,6 +318,13 @@ class GroupedOzaki:
         check(lib().cosma_amd_grouped_ozaki_info(self.ptr, ctypes.byref(mod), ctypes.byref(est), ctypes.byref(tiles),
                                                  ctypes.byref(grid)))
         return {"moduli": mod.value, "error_estimate": est.value, "tiles": tiles.value, "grid": grid.value}
+
+    def check(self, tolerance=1e-14, stream=None):
+        """Error check of the last run: worst expected error, its problem, moduli needed."""
+        w, i, need = ctypes.c_double(), c_i64(), ctypes.c_int()
+        check(lib().cosma_amd_grouped_ozaki_check(self.ptr, tolerance, ctypes.byref(w), ctypes.byref(i),
+                                                  ctypes.byref(need), stream))
+        return {"worst_estimate": w.value, "worst_problem": i.value, "moduli_needed": need.value}
 
     def close(self):
         if self.ptr:

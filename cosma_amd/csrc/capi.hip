@@ -83,6 +83,18 @@ int cosma_amd_handle_create(cosma_amd_handle_t *handle, int device, int streams,
     return COSMA_AMD_OK;
 }
 
+int cosma_amd_handle_get_config(cosma_amd_handle_t h, int *device, int *streams, int64_t *tile_m, int64_t *tile_n,
+                                int64_t *tile_k) {
+    clear_error();
+    CA_CHECK(h != nullptr, "handle_get_config: null handle");
+    if (device) *device = h->device;
+    if (streams) *streams = h->n_streams;
+    if (tile_m) *tile_m = h->tile_m;
+    if (tile_n) *tile_n = h->tile_n;
+    if (tile_k) *tile_k = h->tile_k;
+    return COSMA_AMD_OK;
+}
+
 int cosma_amd_handle_destroy(cosma_amd_handle_t h) {
     clear_error();
     if (!h) return COSMA_AMD_OK;
@@ -327,6 +339,15 @@ int cosma_amd_grouped_ozaki_info(cosma_amd_grouped_ozaki_t plan, int *moduli, do
     CA_CHECK(plan != nullptr, "grouped_ozaki_info: null plan");
     grouped_ozaki_info(plan, moduli, error_estimate, tiles, grid);
     return COSMA_AMD_OK;
+}
+
+int cosma_amd_grouped_ozaki_check(cosma_amd_grouped_ozaki_t plan, double tolerance, double *worst_estimate,
+                                  int64_t *worst_problem, int *moduli_needed, void *stream) {
+    clear_error();
+    CA_CHECK(plan != nullptr, "grouped_ozaki_check: null plan");
+    CA_CHECK(worst_estimate && worst_problem && moduli_needed, "grouped_ozaki_check: null output");
+    return grouped_ozaki_check(plan, tolerance, worst_estimate, worst_problem, moduli_needed,
+                               static_cast<hipStream_t>(stream));
 }
 
 int cosma_amd_grouped_ozaki_destroy(cosma_amd_grouped_ozaki_t plan) {

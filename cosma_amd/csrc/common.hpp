@@ -125,6 +125,8 @@ int grouped_ozaki_graph_run(cosma_amd_grouped_ozaki_t p, hipStream_t s);
 int grouped_ozaki_tune(cosma_amd_grouped_ozaki_t p, hipStream_t s);
 int grouped_ozaki_set_grid(cosma_amd_grouped_ozaki_t p, int grid);
 void grouped_ozaki_info(cosma_amd_grouped_ozaki_t p, int *moduli, double *estimate, int64_t *tiles, int *grid);
+int grouped_ozaki_check(cosma_amd_grouped_ozaki_t p, double tol, double *worst, int64_t *worst_problem,
+                        int *moduli_needed, hipStream_t s);
 int grouped_ozaki_destroy(cosma_amd_grouped_ozaki_t p);
 int grouped_run(cosma_amd_grouped_t p, hipStream_t s);
 int grouped_graph_run(cosma_amd_grouped_t p, hipStream_t s);

@@ -40,6 +40,36 @@ cosma_context<Scalar>::~cosma_context() {
 }
 
 template <typename Scalar>
+void cosma_context<Scalar>::register_state(MPI_Comm comm, const Strategy strategy) {
+    const int P = detail::comm_size(comm);
+    if (P != 1)
+        detail::fail("register_state: the communicator has " + std::to_string(P) + " ranks; this build runs one");
+    const bool same = have_strategy_ && strategy.m == prev_m_ && strategy.n == prev_n_ && strategy.k == prev_k_ &&
+                      strategy.divisors == prev_divisors_ && strategy.split_dimension == prev_dims_ &&
+                      strategy.step_type == prev_types_;
+    if (!same) {
+        memory_pool_.unpin_all();
+        have_strategy_ = true;
+        prev_m_ = strategy.m;
+        prev_n_ = strategy.n;
+        prev_k_ = strategy.k;
+        prev_divisors_ = strategy.divisors;
+        prev_dims_ = strategy.split_dimension;
+        prev_types_ = strategy.step_type;
+    }
+}
+
+template <typename Scalar>
+memory_pool<Scalar> &cosma_context<Scalar>::get_memory_pool() {
+    return memory_pool_;
+}
+
+template <typename Scalar>
+cosma::communicator *cosma_context<Scalar>::get_cosma_comm() {
+    return nullptr;
+}
+
+template <typename Scalar>
 cosma_amd_handle_t cosma_context<Scalar>::get_gpu_context() {
     if (!gpu_ctx_) {
         detail::check(cosma_amd_handle_create(&gpu_ctx_, 0, streams_, tile_m_, tile_n_, tile_k_),
@@ -59,6 +89,7 @@ long long cosma_context<Scalar>::get_cpu_memory_limit() {
 template <typename Scalar>
 void cosma_context<Scalar>::turn_on_output() {
     output = true;
+    memory_pool_.turn_on_output();
 }
 
 template <typename Scalar>

@@ -168,7 +168,20 @@ CosmaMatrix<Scalar>::CosmaMatrix(char label, const Strategy &strategy, int rank,
 
 template <typename Scalar>
 CosmaMatrix<Scalar>::~CosmaMatrix() {
-    if (data_) cosma_amd_host_free(data_);
+    if (allocated_) {
+        auto &pool = ctxt_->get_memory_pool();
+        pool.free_buffer(pool.get_buffer_pointer(buffer_id_), size_);
+    }
+}
+
+template <typename Scalar>
+Scalar *CosmaMatrix<Scalar>::matrix_pointer() {
+    return allocated_ ? ctxt_->get_memory_pool().get_buffer_pointer(buffer_id_) : nullptr;
+}
+
+template <typename Scalar>
+const Scalar *CosmaMatrix<Scalar>::matrix_pointer() const {
+    return allocated_ ? ctxt_->get_memory_pool().get_buffer_pointer(buffer_id_) : nullptr;
 }
 
 template <typename Scalar>
@@ -181,11 +194,12 @@ void CosmaMatrix<Scalar>::build_layout(const Strategy &strategy) {
 
 template <typename Scalar>
 void CosmaMatrix<Scalar>::allocate() {
-    if (data_ || size_ == 0) return;
-    void *p = nullptr;
-    detail::check(cosma_amd_host_alloc(size_ * sizeof(Scalar), &p), "CosmaMatrix");
-    data_ = static_cast<Scalar *>(p);
-    std::fill(data_, data_ + size_, Scalar{0});
+    if (allocated_ || size_ == 0) return;
+    auto &pool = ctxt_->get_memory_pool();
+    buffer_id_ = pool.get_buffer_id(size_);
+    allocated_ = true;
+    Scalar *data = pool.get_buffer_pointer(buffer_id_);
+    std::fill(data, data + size_, Scalar{0});
 }
 
 template <typename Scalar>
@@ -274,6 +288,7 @@ void multiply(cosma_context<Scalar> *ctx, CosmaMatrix<Scalar> &A, CosmaMatrix<Sc
         detail::fail("multiply: matrix shapes do not match the strategy");
     if (!A.matrix_pointer() || !B.matrix_pointer() || !C.matrix_pointer())
         detail::fail("multiply: a matrix has no storage (constructed with dry_run and not allocated)");
+    ctx->register_state(comm, strategy);
     Walk<Scalar> w{ctx, A, B, C, strategy, alpha};
     w.run(0, Range{0, strategy.m}, Range{0, strategy.n}, Range{0, strategy.k}, beta);
 }

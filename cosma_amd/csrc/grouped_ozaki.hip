@@ -82,6 +82,9 @@ struct cosma_amd_grouped_ozaki_s {
     cosma_amd::OzProblem *dev = nullptr;   // descriptors, sorted by decreasing cost
     int *flags = nullptr;                  // one non-finite flag per problem
     unsigned long long *snorm = nullptr;   // two norm bounds per problem (bit patterns)
+    int *emax = nullptr;                   // largest row / column exponent per problem (this run)
+    double *xnorm = nullptr;               // sum of (X'_ij 2^-shift)^2 per problem (this run)
+    std::vector<int64_t> original;         // caller's index of each sorted problem
     int *pbits = nullptr;                  // the bits P of every problem in the current run
     double limit = 0.0;                    // M / 2 of the plan's moduli, with margin
     int *expo = nullptr;                   // exponents E and F of every problem
@@ -115,15 +118,21 @@ __device__ __forceinline__ double vec_at(const double *x, int64_t ld, bool conti
 // the lanes.  snorm[2p + op] receives max ||x_r||^2 2^-2E_r over the vectors
 // (as the bit pattern of a non-negative double; +inf when an exponent lies
 // outside [-NORM_EXP_LIMIT, NORM_EXP_LIMIT], as in the single-problem path).
-__global__ void gz_clear(int *flags, unsigned long long *snorm, int n) {
+__global__ void gz_clear(int *flags, unsigned long long *snorm, int *emax, double *xnorm, int n) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         flags[i] = 0;
         snorm[2 * i] = 0ull;
         snorm[2 * i + 1] = 0ull;
+        emax[2 * i] = ZERO_EXPONENT;
+        emax[2 * i + 1] = ZERO_EXPONENT;
+        xnorm[i] = 0.0;
     }
 }
 
-__global__ __launch_bounds__(256) void gz_exponents(const OzProblem *probs, int *flags, unsigned long long *snorm) {
+// Besides the exponents and norm bounds, emax[2p + op] receives the largest
+// exponent of the problem's rows of op(A) (op = 0) / columns of op(B) (1).
+__global__ __launch_bounds__(256) void gz_exponents(const OzProblem *probs, int *flags, unsigned long long *snorm,
+                                                    int *emax) {
     const OzProblem &q = probs[blockIdx.x];
     if (q.scale_only) return;
     const bool opA = (blockIdx.y & 1) == 0;
@@ -207,6 +216,7 @@ __global__ __launch_bounds__(256) void gz_exponents(const OzProblem *probs, int 
         const int v = threadIdx.x;
         const int64_t r = r0 + v;
         double nrm = 0.0;   // ||x_r||^2 2^-2E_r of this vector
+        int emx = ZERO_EXPONENT;
         if (r < Rp) {
             unsigned long long b;
             double sq;
@@ -228,10 +238,17 @@ __global__ __launch_bounds__(256) void gz_exponents(const OzProblem *probs, int 
             if (r < R && e != ZERO_EXPONENT)
                 nrm = (e < -NORM_EXP_LIMIT || e > NORM_EXP_LIMIT) ? INFINITY : ldexp(sq, -2 * e);
             if (nrm != nrm) nrm = INFINITY;
+            if (r < R && e > emx) emx = e;
         }
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) nrm = fmax(nrm, __shfl_xor(nrm, off));
-        if (v == 0) atomicMax(&snorm[2 * blockIdx.x + (opA ? 0 : 1)], (unsigned long long)__double_as_longlong(nrm));
+        for (int off = 32; off > 0; off >>= 1) {
+            nrm = fmax(nrm, __shfl_xor(nrm, off));
+            emx = max(emx, __shfl_xor(emx, off));
+        }
+        if (v == 0) {
+            atomicMax(&snorm[2 * blockIdx.x + (opA ? 0 : 1)], (unsigned long long)__double_as_longlong(nrm));
+            atomicMax(&emax[2 * blockIdx.x + (opA ? 0 : 1)], emx);
+        }
     }
 }
 
@@ -259,13 +276,16 @@ __global__ void gz_bits(const OzProblem *probs, const int *flags, const unsigned
 // formed, 0 for a zero product and for problems that do not use the moduli
 // (scale-only, or flagged non-finite: those are computed in FP64).  Reads the
 // exponents written by gz_exponents.
-__global__ __launch_bounds__(256) void gz_gamma(const OzProblem *probs, const int *flags, double *gamma) {
+// d0[p] receives sqrt(sum_l ||op(A)(:, l)||^2 ||op(B)(l, :)||^2) 2^-(ea + fb),
+// the size of X the estimate assumes (ea, fb the largest exponents).
+__global__ __launch_bounds__(256) void gz_gamma(const OzProblem *probs, const int *flags, double *gamma,
+                                                double *d0) {
     const OzProblem &q = probs[blockIdx.x];
     __shared__ int emax[2];
     __shared__ double part[5][256];
     const int t = threadIdx.x;
     if (q.scale_only || flags[blockIdx.x]) {
-        if (t == 0) gamma[blockIdx.x] = 0.0;
+        if (t == 0) gamma[blockIdx.x] = d0[blockIdx.x] = 0.0;
         return;
     }
     if (t < 2) emax[t] = INT_MIN;
@@ -279,7 +299,7 @@ __global__ __launch_bounds__(256) void gz_gamma(const OzProblem *probs, const in
     ea = emax[0];
     fb = emax[1];
     if (ea == ZERO_EXPONENT || fb == ZERO_EXPONENT) {
-        if (t == 0) gamma[blockIdx.x] = 0.0;
+        if (t == 0) gamma[blockIdx.x] = d0[blockIdx.x] = 0.0;
         return;
     }
     double SA = 0.0, SB = 0.0, NA = 0.0, NB = 0.0, D = 0.0;
@@ -314,6 +334,7 @@ __global__ __launch_bounds__(256) void gz_gamma(const OzProblem *probs, const in
         const double d = part[4][0];
         const double g = sqrt((part[0][0] * part[3][0] + part[2][0] * part[1][0]) / (12.0 * d));
         gamma[blockIdx.x] = (d > 0.0 && isfinite(g)) ? g : INFINITY;
+        d0[blockIdx.x] = sqrt(d);
     }
 }
 
@@ -496,13 +517,18 @@ __global__ __launch_bounds__(256) void gz_products(const OzProblem *probs, int n
 // Recombination and epilogue: workgroup row blockIdx.x = problem, a thread
 // takes 4 consecutive rows of one column.  Problems with k == 0 or alpha == 0
 // only scale C; problems flagged non-finite are computed by FP64 dot products.
+// xnorm[p] receives the sum of (X'_ij 2^-shift)^2 over the problem's
+// recombined products X' (before alpha and beta), shift = ea + fb +
+// ceil(log2(k + 1)) from emax, for the error check after a run.
 template <int NL>
 __global__ __launch_bounds__(256) void gz_recombine(const OzProblem *probs, const int *flags, const int *pbits,
-                                                    CrtConst cc) {
+                                                    CrtConst cc, const int *emax, double *xnorm) {
     const OzProblem &q = probs[blockIdx.x];
     const int P = pbits[blockIdx.x];
     const int64_t quads = (q.m + 3) / 4, items = quads * q.n;
     const bool plain = flags[blockIdx.x] != 0;
+    const int shift = emax[2 * blockIdx.x] + emax[2 * blockIdx.x + 1] + (int)ceil(log2((double)q.k + 1.0));
+    double ss = 0.0;
     for (int64_t it = threadIdx.x + (int64_t)blockIdx.y * blockDim.x; it < items;
          it += (int64_t)blockDim.x * gridDim.y) {
         const int64_t j = it / quads, i0 = (it - j * quads) * 4;
@@ -536,13 +562,25 @@ __global__ __launch_bounds__(256) void gz_recombine(const OzProblem *probs, cons
             const int fj = q.F[j];
 #pragma unroll
             for (int e = 0; e < 4; ++e)
-                if (i0 + e < q.m) val[e] = q.alpha * ldexp(crt_value<NL>(s[e], cc), q.E[i0 + e] + fj - 2 * P);
+                if (i0 + e < q.m) {
+                    const double v = crt_value<NL>(s[e], cc);
+                    const double x = ldexp(v, q.E[i0 + e] + fj - 2 * P - shift);
+                    ss = fma(x, x, ss);
+                    val[e] = q.alpha * ldexp(v, q.E[i0 + e] + fj - 2 * P);
+                }
         }
         for (int e = 0; e < 4 && i0 + e < q.m; ++e) {
             double *cp = q.c + i0 + e + j * q.ldc;
             *cp = (q.beta == 0.0) ? val[e] : fma(q.beta, *cp, val[e]);
         }
     }
+    // one atomic add per workgroup
+    __shared__ double part[4];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = ss;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(&xnorm[blockIdx.x], part[0] + part[1] + part[2] + part[3]);
 }
 
 constexpr int GZ_RECOMBINE_Y = 8;
@@ -576,6 +614,8 @@ int grouped_ozaki_destroy(cosma_amd_grouped_ozaki_t p) {
     if (p->flags) (void)hipFree(p->flags);
     if (p->snorm) (void)hipFree(p->snorm);
     if (p->pbits) (void)hipFree(p->pbits);
+    if (p->emax) (void)hipFree(p->emax);
+    if (p->xnorm) (void)hipFree(p->xnorm);
     if (p->expo) (void)hipFree(p->expo);
     if (p->ws) (void)hipFree(p->ws);
     delete p;
@@ -587,8 +627,8 @@ int grouped_ozaki_run(cosma_amd_grouped_ozaki_t p, hipStream_t s) {
     CA_HIP(hipSetDevice(p->h->device));
     const dim3 gv((unsigned)p->nprob, (unsigned)(2 * p->vgroups));
     const dim3 g1((unsigned)((p->nprob + 255) / 256));
-    hipLaunchKernelGGL(gz_clear, g1, dim3(256), 0, s, p->flags, p->snorm, p->nprob);
-    hipLaunchKernelGGL(gz_exponents, gv, dim3(256), 0, s, p->dev, p->flags, p->snorm);
+    hipLaunchKernelGGL(gz_clear, g1, dim3(256), 0, s, p->flags, p->snorm, p->emax, p->xnorm, p->nprob);
+    hipLaunchKernelGGL(gz_exponents, gv, dim3(256), 0, s, p->dev, p->flags, p->snorm, p->emax);
     hipLaunchKernelGGL(gz_bits, g1, dim3(256), 0, s, p->dev, p->flags, p->snorm, p->pbits, p->nprob, p->limit);
     hipLaunchKernelGGL(gz_split, gv, dim3(256), 0, s, p->dev, p->pbits, p->md);
     if (p->total_tiles > 0)
@@ -596,9 +636,11 @@ int grouped_ozaki_run(cosma_amd_grouped_ozaki_t p, hipStream_t s) {
                            p->md);
     const dim3 gr((unsigned)p->nprob, GZ_RECOMBINE_Y);
     if (crt_limbs(p->moduli) == 3)
-        hipLaunchKernelGGL(gz_recombine<3>, gr, dim3(256), 0, s, p->dev, p->flags, p->pbits, p->cc);
+        hipLaunchKernelGGL(gz_recombine<3>, gr, dim3(256), 0, s, p->dev, p->flags, p->pbits, p->cc, p->emax,
+                           p->xnorm);
     else
-        hipLaunchKernelGGL(gz_recombine<NLIMB>, gr, dim3(256), 0, s, p->dev, p->flags, p->pbits, p->cc);
+        hipLaunchKernelGGL(gz_recombine<NLIMB>, gr, dim3(256), 0, s, p->dev, p->flags, p->pbits, p->cc, p->emax,
+                           p->xnorm);
     CA_HIP(hipGetLastError());
     return COSMA_AMD_OK;
 }
@@ -674,6 +716,7 @@ int grouped_ozaki_create(cosma_amd_handle_t h, const cosma_amd_gemm_desc *d, int
     CA_HIP(hipSetDevice(h->device));
     std::vector<OzProblem> probs;
     std::vector<double> cost;
+    std::vector<int64_t> orig;
     for (int64_t i = 0; i < count; ++i) {
         const cosma_amd_gemm_desc &q = d[i];
         const char ta = norm_trans(q.transa), tb = norm_trans(q.transb);
@@ -717,6 +760,7 @@ int grouped_ozaki_create(cosma_amd_handle_t h, const cosma_amd_gemm_desc *d, int
         g.ldr = up(q.m, 16);
         probs.push_back(g);
         cost.push_back(scale_only ? 0.0 : (double)q.m * q.n * q.k);
+        orig.push_back(i);
     }
     std::vector<int> order(probs.size());
     std::iota(order.begin(), order.end(), 0);
@@ -729,6 +773,7 @@ int grouped_ozaki_create(cosma_amd_handle_t h, const cosma_amd_gemm_desc *d, int
     p->h = h;
     p->nprob = (int)sorted.size();
     p->grid = 1;
+    for (int i : order) p->original.push_back(orig[i]);
     if (p->nprob == 0) {
         p->moduli = moduli ? moduli : 2;
         *out = p;
@@ -766,6 +811,8 @@ int grouped_ozaki_create(cosma_amd_handle_t h, const cosma_amd_gemm_desc *d, int
     if ((rc = alloc(reinterpret_cast<void **>(&p->snorm), 2 * sorted.size() * sizeof(unsigned long long))))
         return fail(rc);
     if ((rc = alloc(reinterpret_cast<void **>(&p->pbits), sorted.size() * sizeof(int)))) return fail(rc);
+    if ((rc = alloc(reinterpret_cast<void **>(&p->emax), 2 * sorted.size() * sizeof(int)))) return fail(rc);
+    if ((rc = alloc(reinterpret_cast<void **>(&p->xnorm), sorted.size() * sizeof(double)))) return fail(rc);
 
     // exponents first (their space does not depend on the moduli count)
     int64_t nexp = 0;
@@ -795,12 +842,13 @@ int grouped_ozaki_create(cosma_amd_handle_t h, const cosma_amd_gemm_desc *d, int
     std::vector<unsigned long long> hnorm(2 * sorted.size());
     {
         double *dgamma = nullptr;
-        if ((rc = alloc(reinterpret_cast<void **>(&dgamma), sorted.size() * sizeof(double)))) return fail(rc);
+        if ((rc = alloc(reinterpret_cast<void **>(&dgamma), 2 * sorted.size() * sizeof(double)))) return fail(rc);
         const dim3 gv((unsigned)p->nprob, (unsigned)(2 * p->vgroups));
         hipLaunchKernelGGL(gz_clear, dim3((unsigned)((p->nprob + 255) / 256)), dim3(256), 0, 0, p->flags, p->snorm,
-                           p->nprob);
-        hipLaunchKernelGGL(gz_exponents, gv, dim3(256), 0, 0, p->dev, p->flags, p->snorm);
-        hipLaunchKernelGGL(gz_gamma, dim3((unsigned)p->nprob), dim3(256), 0, 0, p->dev, p->flags, dgamma);
+                           p->emax, p->xnorm, p->nprob);
+        hipLaunchKernelGGL(gz_exponents, gv, dim3(256), 0, 0, p->dev, p->flags, p->snorm, p->emax);
+        hipLaunchKernelGGL(gz_gamma, dim3((unsigned)p->nprob), dim3(256), 0, 0, p->dev, p->flags, dgamma,
+                           dgamma + sorted.size());
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) e = hipMemcpy(gamma.data(), dgamma, gamma.size() * sizeof(double), hipMemcpyDeviceToHost);
         if (e == hipSuccess)
@@ -878,6 +926,78 @@ int grouped_ozaki_create(cosma_amd_handle_t h, const cosma_amd_gemm_desc *d, int
     crt_constants(moduli, p->cc);
     p->grid = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, (int64_t)h->n_cu * 8));
     *out = p;
+    return COSMA_AMD_OK;
+}
+
+int grouped_ozaki_check(cosma_amd_grouped_ozaki_t p, double tol, double *worst, int64_t *worst_problem,
+                        int *moduli_needed, hipStream_t s) {
+    if (tol <= 0.0) tol = 1e-14;
+    *worst = 0.0;
+    *worst_problem = -1;
+    *moduli_needed = p->moduli;
+    if (p->nprob == 0) return COSMA_AMD_OK;
+    CA_HIP(hipSetDevice(p->h->device));
+    const size_t n = (size_t)p->nprob;
+    double *dg = nullptr;
+    if (hipMalloc(reinterpret_cast<void **>(&dg), 2 * n * sizeof(double)) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("grouped ozaki: check: allocation failed");
+        return COSMA_AMD_ERR_ALLOC;
+    }
+    std::vector<double> gamma(n), d0(n), xn(n);
+    std::vector<int> pb(n), em(2 * n), fl(n);
+    std::vector<unsigned long long> sn(2 * n);
+    std::vector<OzProblem> desc(n);
+    hipLaunchKernelGGL(gz_gamma, dim3((unsigned)n), dim3(256), 0, s, p->dev, p->flags, dg, dg + n);
+    hipError_t e = hipGetLastError();
+    auto get = [&](void *dst, const void *src, size_t bytes) {
+        if (e == hipSuccess) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s);
+    };
+    get(gamma.data(), dg, n * sizeof(double));
+    get(d0.data(), dg + n, n * sizeof(double));
+    get(xn.data(), p->xnorm, n * sizeof(double));
+    get(pb.data(), p->pbits, n * sizeof(int));
+    get(em.data(), p->emax, 2 * n * sizeof(int));
+    get(fl.data(), p->flags, n * sizeof(int));
+    get(sn.data(), p->snorm, 2 * n * sizeof(unsigned long long));
+    get(desc.data(), p->dev, n * sizeof(OzProblem));
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(dg);
+    if (e != hipSuccess) return hip_fail(e, "grouped ozaki check", __FILE__, __LINE__);
+    // expected relative error of problem i with P bits, relative to its X'
+    auto est = [&](size_t i, int P) {
+        const double nx = std::sqrt(xn[i]) * std::ldexp(1.0, (int)std::ceil(std::log2((double)desc[i].k + 1.0)));
+        return nx > 0.0 ? gamma[i] * std::ldexp(1.0, -P) * d0[i] / nx : (double)INFINITY;
+    };
+    auto uses_moduli = [&](size_t i) { return !desc[i].scale_only && !fl[i] && gamma[i] != 0.0; };
+    for (size_t i = 0; i < n; ++i) {
+        if (!uses_moduli(i)) continue;
+        const double v = est(i, pb[i]);
+        if (!(v <= *worst)) {
+            *worst = v;
+            *worst_problem = p->original[i];
+        }
+    }
+    // the smallest count whose bits meet the tolerance with 1.5 x the estimate
+    // for every problem (the single-problem check's factor)
+    int need = 2;
+    for (; need <= MAX_MODULI; ++need) {
+        const double limit = moduli_half_limit(need);
+        bool ok = true;
+        for (size_t i = 0; i < n && ok; ++i) {
+            if (!uses_moduli(i)) continue;
+            int P = std::min(moduli_bits(need, desc[i].k), MAX_P);
+            if (P < 1) { ok = false; break; }
+            double na, nb;
+            std::memcpy(&na, &sn[2 * i], sizeof(double));
+            std::memcpy(&nb, &sn[2 * i + 1], sizeof(double));
+            P = norm_bits(limit, 0.5 * std::sqrt((double)desc[i].k), std::sqrt(na) * (1.0 + 1e-9),
+                          std::sqrt(nb) * (1.0 + 1e-9), P);
+            ok = 1.5 * est(i, P) <= tol;
+        }
+        if (ok) break;
+    }
+    *moduli_needed = need;   // MAX_MODULI + 1: no count meets the tolerance
     return COSMA_AMD_OK;
 }
 

@@ -9,10 +9,10 @@
 // dimension; with no such steps it is the whole matrix, column-major with
 // ld = rows (src/cosma/mapper.cpp:105-200).
 //
-// The storage is pinned host memory (hipHostMalloc through the cosma_amd
-// C ABI), so the host-streaming GEMM copies it by DMA without staging; this
-// replaces the reference's memory pool plus in-place pinning
-// (src/cosma/memory_pool.cpp:153-165, src/cosma/pinned_buffers.cpp:6-47).
+// The storage is a buffer of the context's memory pool (page-locked host
+// memory unless COSMA_GPU_MEMORY_PINNING=OFF), as in the reference
+// (src/cosma/buffer.cpp:131-141): its address is looked up in the pool on
+// every matrix_pointer(), because the pool may move when it grows.
 #pragma once
 #include <cstddef>
 #include <iosfwd>
@@ -51,12 +51,12 @@ class CosmaMatrix {
     char which_matrix() const { return label_; }
     int rank() const { return rank_; }
 
-    scalar_t *matrix_pointer() { return data_; }
-    const scalar_t *matrix_pointer() const { return data_; }
+    scalar_t *matrix_pointer();
+    const scalar_t *matrix_pointer() const;
     std::size_t matrix_size() const { return size_; }
     std::size_t matrix_size(int rank) const { return rank == 0 ? size_ : 0; }
-    scalar_t &operator[](std::size_t index) { return data_[index]; }
-    scalar_t operator[](std::size_t index) const { return data_[index]; }
+    scalar_t &operator[](std::size_t index) { return matrix_pointer()[index]; }
+    scalar_t operator[](std::size_t index) const { return matrix_pointer()[index]; }
 
     // (local index, owning rank) of global element (gi, gj)
     std::pair<int, int> local_coordinates(int gi, int gj) const;
@@ -83,7 +83,8 @@ class CosmaMatrix {
     int m_ = 0, n_ = 0;
     std::vector<LocalBlock> blocks_;
     std::size_t size_ = 0;
-    scalar_t *data_ = nullptr;
+    bool allocated_ = false;
+    std::size_t buffer_id_ = 0;   // offset of the matrix in the context's memory pool
 };
 
 template <typename Scalar>

@@ -53,6 +53,13 @@ typedef struct cosma_amd_handle_s *cosma_amd_handle_t;
 int cosma_amd_handle_create(cosma_amd_handle_t *handle, int device, int streams,
                             int64_t tile_m, int64_t tile_n, int64_t tile_k);
 int cosma_amd_handle_destroy(cosma_amd_handle_t handle);
+/*
+ * The device, compute-stream count and streaming tile sizes of a handle
+ * (after the defaults were applied).  Diagnostic; Tiled-MM's handle keeps
+ * the same settings, but COSMA's call sites never read them back.
+ */
+int cosma_amd_handle_get_config(cosma_amd_handle_t handle, int *device, int *streams, int64_t *tile_m,
+                                int64_t *tile_n, int64_t *tile_k);
 
 /* ----------------------------------------------------- device-resident GEMM */
 /*
@@ -316,6 +323,21 @@ int cosma_amd_grouped_ozaki_tune(cosma_amd_grouped_ozaki_t plan, void *stream);
 int cosma_amd_grouped_ozaki_set_grid(cosma_amd_grouped_ozaki_t plan, int grid);
 int cosma_amd_grouped_ozaki_info(cosma_amd_grouped_ozaki_t plan, int *moduli, double *error_estimate,
                                  int64_t *tiles, int *grid);
+/*
+ * Checks the last run of the plan against `tolerance` (<= 0: 1e-14): for
+ * every problem the expected relative error of its computed product X'
+ * (before alpha and beta), with the norm of X' summed in that run's
+ * recombination and the error factors of the operands now present.  Returns
+ * the largest of these estimates, the caller's index of that problem (-1 if
+ * no problem uses the moduli), and the smallest moduli count for which
+ * 1.5 x every estimate meets the tolerance (MAX + 1 = 21 if none does).  The
+ * plan's count is fixed (its graph is); when moduli_needed exceeds it, the
+ * operands have changed (or cancel) more than the count chosen at creation
+ * allows, and a plan created with moduli_needed computes them to the
+ * tolerance.  Synchronises `stream`; call it after a run on the same stream.
+ */
+int cosma_amd_grouped_ozaki_check(cosma_amd_grouped_ozaki_t plan, double tolerance, double *worst_estimate,
+                                  int64_t *worst_problem, int *moduli_needed, void *stream);
 int cosma_amd_grouped_ozaki_destroy(cosma_amd_grouped_ozaki_t plan);
 
 /* ------------------------------------------------------ pinned host memory */

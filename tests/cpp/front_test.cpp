@@ -12,6 +12,14 @@
 //       cosma::multiply with CosmaMatrix objects and the sequential steps
 //       given as "sm2,sk3" ("-" for none).  Matrices hold the same global
 //       values as above; writes C in global column-major order.
+//   front_test env [handle]
+//       The knobs as the interface reads them (one "name value" line each);
+//       with "handle" also the streams and tiles of the context's device
+//       handle (needs the GPU).
+//   front_test pool
+//       memory_pool and CosmaMatrix storage checks (no GPU needed with
+//       COSMA_GPU_MEMORY_PINNING=OFF); prints "ok ..." / "FAIL ..." lines and
+//       the pool's figures.
 //   front_test pdgemm ta tb m n k alpha beta ia ja ib jb ic jc out.bin
 //       cosma_pdgemm on a 1 x 1 grid: global matrices with 2 extra rows and
 //       columns around the submatrices (seeds 1, 2, 3 over the whole local
@@ -26,7 +34,9 @@
 #include <string>
 #include <vector>
 
+#include <cosma/environment_variables.hpp>
 #include <cosma/local_multiply.hpp>
+#include <cosma/memory_pool.hpp>
 #include <cosma/multiply.hpp>
 #include <cosma/prefixed_pxgemm.h>
 
@@ -96,6 +106,96 @@ int run_strategy() {
                 std::cout << " " << b.row0 << ":" << b.rows << ":" << b.col0 << ":" << b.cols << "@" << b.offset;
             std::cout << "\n";
         }
+    }
+    return 0;
+}
+
+int run_env(int argc, char **argv) {
+    cosma::cosma_context<double> ctx;
+    std::cout << "streams " << ctx.get_gpu_streams() << "\n";
+    std::cout << "tiles " << ctx.get_gpu_tile_m() << " " << ctx.get_gpu_tile_n() << " " << ctx.get_gpu_tile_k() << "\n";
+    std::cout << "cpu_max_memory " << ctx.get_cpu_memory_limit() << "\n";
+    std::cout << "pinning " << ctx.pin_host_buffers << " " << ctx.get_memory_pool().page_locked() << "\n";
+    std::cout << "amortization " << ctx.get_memory_pool().amortization << "\n";
+    std::cout << "alignment " << cosma::get_cosma_cpu_memory_alignment() << "\n";
+    std::cout << "dim_threshold " << cosma::get_cosma_dim_threshold() << "\n";
+    std::cout << "adapt " << ctx.adapt_to_scalapack_strategy << "\n";
+    std::cout << "overlap " << ctx.overlap_comm_and_comp << "\n";
+    std::cout << "unified " << ctx.unified_memory() << "\n";
+    std::cout << "min_local_dimension " << cosma::get_min_local_dimension() << "\n";
+    if (argc > 2 && std::string(argv[2]) == "handle") {
+        int dev = 0, st = 0;
+        int64_t tm = 0, tn = 0, tk = 0;
+        if (cosma_amd_handle_get_config(ctx.get_gpu_context(), &dev, &st, &tm, &tn, &tk) != COSMA_AMD_OK)
+            throw std::runtime_error(cosma_amd_last_error());
+        std::cout << "handle " << dev << " " << st << " " << tm << " " << tn << " " << tk << "\n";
+    }
+    return 0;
+}
+
+void expect(bool ok, const std::string &what) { std::cout << (ok ? "ok " : "FAIL ") << what << "\n"; }
+
+int run_pool() {
+    // a pool of its own: buffers, growth, order of release
+    {
+        cosma::memory_pool<double> pool;
+        const size_t pad = pool.padded(1001);
+        std::vector<size_t> sizes = {1001, 500, 3};
+        pool.reserve(sizes);
+        std::cout << "reserved " << pool.capacity() << " padded1001 " << pad << " sizes " << sizes[0] << " "
+                  << sizes[1] << " " << sizes[2] << "\n";
+        const size_t a = pool.get_buffer_id(1001), b = pool.get_buffer_id(500), c = pool.get_buffer_id(3);
+        std::cout << "ids " << a << " " << b << " " << c << " size " << pool.size() << "\n";
+        double *pa = pool.get_buffer_pointer(a);
+        for (int i = 0; i < 1001; ++i) pa[i] = i;
+        double *pb = pool.get_buffer_pointer(b);
+        for (int i = 0; i < 500; ++i) pb[i] = -i;
+        const size_t cap0 = pool.capacity();
+        expect(cap0 >= pool.size(), "reserve covers the buffers");
+        // a buffer beyond the capacity grows the pool and keeps the contents
+        const size_t big = pool.get_buffer_id(10 * cap0);
+        double *pbig = pool.get_buffer_pointer(big);
+        pbig[0] = 7.0;
+        pa = pool.get_buffer_pointer(a);
+        pb = pool.get_buffer_pointer(b);
+        bool kept = true;
+        for (int i = 0; i < 1001; ++i) kept &= pa[i] == i;
+        for (int i = 0; i < 500; ++i) kept &= pb[i] == -i;
+        expect(pool.capacity() >= pool.size() && pool.capacity() > cap0, "growth beyond the reserve");
+        expect(kept, "growth keeps the contents");
+        // release out of order: b before the buffers above it
+        const size_t top_before = pool.size();
+        pool.free_buffer(pool.get_buffer_pointer(b), 500);
+        expect(pool.size() == top_before, "a buffer below the top stays until the top is released");
+        pool.free_buffer(pool.get_buffer_pointer(big), 10 * cap0);
+        expect(pool.size() == c + pool.padded(3), "releasing the top stops at c, still in use");
+        pool.free_buffer(pool.get_buffer_pointer(c), 3);
+        expect(pool.size() == a + pool.padded(1001), "the released b goes with c");
+        pool.free_buffer(pool.get_buffer_pointer(a), 1001);
+        expect(pool.size() == 0, "empty");
+        std::cout << "pool capacity " << pool.capacity() << " page_locked " << pool.page_locked() << "\n";
+    }
+    // CosmaMatrix storage comes from the context's pool
+    {
+        auto ctx = cosma::make_context<double>();
+        cosma::Strategy s(30, 20, 40, 1);
+        {
+            cosma::CosmaMatrix<double> A(ctx, 'A', s, 0), B(ctx, 'B', s, 0);
+            for (size_t i = 0; i < A.matrix_size(); ++i) A.matrix_pointer()[i] = (double)i;
+            cosma::CosmaMatrix<double> C(ctx, 'C', s, 0);   // may move the pool
+            bool ok = true;
+            for (size_t i = 0; i < A.matrix_size(); ++i) ok &= A[i] == (double)i;
+            expect(ok, "matrix contents survive the pool growing");
+            auto &pool = ctx->get_memory_pool();
+            expect(pool.size() == pool.padded(1200) + pool.padded(800) + pool.padded(600), "three matrices in the pool");
+            expect(A.matrix_pointer() == pool.get_pool_pointer(), "A at the bottom of the pool");
+            std::cout << "matrices " << A.matrix_size() << " " << B.matrix_size() << " " << C.matrix_size()
+                      << " pool " << pool.size() << "\n";
+        }
+        expect(ctx->get_memory_pool().size() == 0, "matrices return their buffers");
+        expect(ctx->get_cosma_comm() == nullptr, "no communicator on one rank");
+        ctx->register_state(0, s);
+        expect(!ctx->get_memory_pool().already_pinned, "register_state of a new strategy");
     }
     return 0;
 }
@@ -186,6 +286,8 @@ int main(int argc, char **argv) {
     try {
         const std::string cmd = argv[1];
         if (cmd == "strategy") return run_strategy();
+        if (cmd == "env") return run_env(argc, argv);
+        if (cmd == "pool") return run_pool();
         if (cmd == "local_multiply") return run_local_multiply(argc, argv);
         if (cmd == "multiply") return run_multiply(argc, argv);
         if (cmd == "pdgemm") return run_pdgemm(argc, argv);

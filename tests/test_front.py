@@ -124,3 +124,93 @@ def test_layouts_tile_exactly():
                 for j in range(c0, c0 + nc):
                     cover[i][j] += 1
         assert all(v == 1 for row in cover for v in row), (steps, label)
+
+
+# ----------------------------------------------------------- environment knobs
+KNOBS = {
+    "COSMA_GPU_STREAMS": "5", "COSMA_GPU_MAX_TILE_M": "1000", "COSMA_GPU_MAX_TILE_N": "2000",
+    "COSMA_GPU_MAX_TILE_K": "3000", "COSMA_GPU_MEMORY_PINNING": "off", "COSMA_CPU_MAX_MEMORY": "1",
+    "COSMA_MEMORY_POOL_AMORTIZATION": "1.5", "COSMA_CPU_MEMORY_ALIGNMENT": "256", "COSMA_DIM_THRESHOLD": "77",
+    "COSMA_ADAPT_STRATEGY": "OFF", "COSMA_OVERLAP_COMM_AND_COMP": "on", "COSMA_GPU_UNIFIED_MEMORY": "ON",
+    "COSMA_MIN_LOCAL_DIMENSION": "33",
+}
+
+
+def run_driver(args, env_extra=None, drop_cosma=True, check=True):
+    env = {k: v for k, v in os.environ.items() if not (drop_cosma and k.startswith("COSMA_"))}
+    env.update(env_extra or {})
+    r = subprocess.run([driver(), *args], capture_output=True, text=True, timeout=120, env=env)
+    if check:
+        assert r.returncode == 0, (r.stdout, r.stderr)
+    return r
+
+
+def env_values(env_extra=None):
+    out = run_driver(["env"], env_extra).stdout
+    return {l.split()[0]: l.split()[1:] for l in out.splitlines()}
+
+
+def test_knob_defaults():
+    """Defaults of src/cosma/environment_variables.hpp:10-97."""
+    v = env_values()
+    assert v["streams"] == ["2"] and v["tiles"] == ["5000", "5000", "5000"]
+    assert v["cpu_max_memory"] == [str(2 ** 63 - 1)]
+    assert v["pinning"] == ["1", "1"]               # pinned and the pool is page-locked
+    assert v["amortization"] == ["1.2"] and v["alignment"] == ["0"] and v["dim_threshold"] == ["0"]
+    assert v["adapt"] == ["1"] and v["overlap"] == ["0"] and v["unified"] == ["0"]
+    assert v["min_local_dimension"] == ["200"]
+
+
+def test_knobs_set():
+    """Every knob read with the reference's parsing: numbers with atoi/atof,
+    COSMA_CPU_MAX_MEMORY in MB converted to elements, booleans true only for
+    "ON" in any case (src/cosma/environment_variables.cpp:9-21)."""
+    v = env_values(KNOBS)
+    assert v["streams"] == ["5"] and v["tiles"] == ["1000", "2000", "3000"]
+    assert v["cpu_max_memory"] == [str(1024 * 1024 // 8)]
+    assert v["pinning"] == ["0", "0"]               # "off" is false: pageable pool
+    assert v["amortization"] == ["1.5"] and v["alignment"] == ["256"] and v["dim_threshold"] == ["77"]
+    assert v["adapt"] == ["0"] and v["overlap"] == ["1"] and v["unified"] == ["1"]
+    assert v["min_local_dimension"] == ["33"]
+    # only "ON" (any case) is true
+    assert env_values({"COSMA_GPU_MEMORY_PINNING": "1"})["pinning"] == ["0", "0"]
+    assert env_values({"COSMA_ADAPT_STRATEGY": "yes"})["adapt"] == ["0"]
+    assert env_values({"COSMA_OVERLAP_COMM_AND_COMP": "oN"})["overlap"] == ["1"]
+
+
+def test_cpu_max_memory_limits_pdgemm():
+    """pdgemm needs m*k + k*n + m*n elements on one rank; with
+    COSMA_CPU_MAX_MEMORY (MB) below that it throws the reference's message
+    (src/cosma/strategy.cpp:373-381) before touching the device."""
+    r = run_driver(["pdgemm", "N", "N", "300", "300", "300", "1", "0", "1", "1", "1", "1", "1", "1", "/dev/null"],
+                   {"COSMA_CPU_MAX_MEMORY": "1"}, check=False)
+    assert r.returncode == 1
+    assert "requires the memory for at least 270000 units, but only 131072 units are allowed" in r.stderr
+
+
+def pool_lines(env_extra):
+    out = run_driver(["pool"], env_extra).stdout.splitlines()
+    assert not [l for l in out if l.startswith("FAIL")], out
+    return {l.split()[0]: l.split()[1:] for l in out if not l.startswith("ok ")}
+
+
+def test_memory_pool_amortization_and_release_order():
+    """reserve() takes the buffers times COSMA_MEMORY_POOL_AMORTIZATION
+    (src/cosma/memory_pool.cpp:116-150): 1504 elements x 1.2 = 1805; buffers
+    are a stack, one released below the top goes when the top does; growth
+    keeps the contents; CosmaMatrix objects take their storage from the
+    context's pool and give it back."""
+    v = pool_lines({"COSMA_GPU_MEMORY_PINNING": "OFF"})
+    assert v["reserved"][:1] == ["1805"] and v["ids"] == ["0", "1001", "1501", "size", "1504"]
+    assert v["matrices"] == ["1200", "800", "600", "pool", "2600"]
+    assert v["pool"][-1] == "0"        # pageable storage with pinning off
+
+
+def test_memory_pool_alignment():
+    """COSMA_CPU_MEMORY_ALIGNMENT = 256 bytes: every buffer is padded to a
+    multiple of 32 doubles, and the amortised reserve too."""
+    v = pool_lines({"COSMA_GPU_MEMORY_PINNING": "OFF", "COSMA_CPU_MEMORY_ALIGNMENT": "256",
+                    "COSMA_MEMORY_POOL_AMORTIZATION": "1.5"})
+    assert v["reserved"] == ["2368", "padded1001", "1024", "sizes", "1024", "512", "32"]
+    assert v["ids"] == ["0", "1024", "1536", "size", "1568"]
+    assert v["matrices"] == ["1200", "800", "600", "pool", str(1216 + 800 + 608)]

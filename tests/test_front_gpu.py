@@ -189,3 +189,44 @@ def test_dmultiply_using_layout(ta, tb):
                                ctypes.byref(ctypes.c_double(0.5)), ctypes.byref(lc))
     got = gather(cbufs, rc, cc, (m, n))
     assert oracle.rel_fro(got, expected(ta, tb, m, n, k, -1.5, A, B, 0.5, C0)) <= gemm_bound(k)
+
+
+def _driver_env(args, env_extra):
+    env = {k: v for k, v in os.environ.items() if not k.startswith("COSMA_")}
+    env.update(env_extra)
+    print("front_test", *args, env_extra, flush=True)
+    r = subprocess.run([cbuild.TEST_DRIVER, *args], capture_output=True, text=True, timeout=120, env=env)
+    assert r.returncode == 0, (r.stdout, r.stderr)
+    return r.stdout.splitlines()
+
+
+def test_knobs_reach_the_device_handle():
+    """COSMA_GPU_STREAMS and COSMA_GPU_MAX_TILE_{M,N,K} become the streams
+    and streaming tiles of the context's device handle."""
+    lines = _driver_env(["env", "handle"], {"COSMA_GPU_STREAMS": "3", "COSMA_GPU_MAX_TILE_M": "700",
+                                            "COSMA_GPU_MAX_TILE_N": "900", "COSMA_GPU_MAX_TILE_K": "1100"})
+    assert "handle 0 3 700 900 1100" in lines, lines
+    assert "handle 0 2 5000 5000 5000" in _driver_env(["env", "handle"], {})
+
+
+def test_memory_pool_page_locked():
+    """With COSMA_GPU_MEMORY_PINNING on (the default) the pool's storage is
+    page-locked host memory, with the same stack and growth behaviour."""
+    lines = _driver_env(["pool"], {})
+    assert not [l for l in lines if l.startswith("FAIL")], lines
+    assert [l for l in lines if l.startswith("pool capacity")][0].endswith("page_locked 1")
+
+
+def test_multiply_from_pageable_pool():
+    """COSMA_GPU_MEMORY_PINNING=OFF: matrices in pageable memory, staged by the
+    streaming GEMM; same results as from the page-locked pool."""
+    import tempfile as tf
+    m, n, k = 150, 130, 170
+    with tf.TemporaryDirectory() as d:
+        outs = []
+        for pin in ("ON", "OFF"):
+            out = os.path.join(d, f"c_{pin}.bin")
+            _driver_env(["multiply", str(m), str(n), str(k), "sm2,sk2", "1.0", "0.5", out],
+                        {"COSMA_GPU_MEMORY_PINNING": pin})
+            outs.append(np.fromfile(out, dtype=np.float64))
+    assert np.array_equal(outs[0], outs[1])
