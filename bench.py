@@ -1,30 +1,50 @@
 """Benchmark of the single-rank local multiply on one MI355X.
 
-Workload (the plain-FP64 configuration the headline metric is quoted on):
-C <- alpha * A^T * B + beta * C with m = n = k = 16384, FP64, A, B, C uniform
-[-1, 1) (seeded SplitMix64, generated in HBM), alpha = 1, beta = 0.5,
-operands device-resident.  One step = one DGEMM call through the C ABI.
+Headline (`value`): C <- alpha * A^T * B + beta * C with m = n = k = 16384,
+FP64, A, B, C uniform [-1, 1) (seeded SplitMix64, generated in HBM),
+alpha = 1, beta = 0.5, operands device-resident.  One step = one DGEMM call
+through the C ABI; the line carries the kernel's roofline fraction (HIP
+events on its launch stream), the error of the result against the
+double-double oracle on sampled rows, and the reference's CPU path (one
+host-BLAS cblas_dgemm, src/cosma/blas.cpp:35) timed on this machine's cores.
 
-Prints one JSON line with the throughput (TFLOP/s = 2mnk / t), the measured
-roofline fraction of the DGEMM kernel (FP64 MFMA peak), the error of the
-result against the double-double oracle on sampled rows, and the reference's
-CPU path (one host-BLAS cblas_dgemm, src/cosma/blas.cpp:35) timed on a
-bounded sample on this machine's cores.
+The same JSON line carries every other benchmark configuration as a
+sub-object with its own time, error and roofline (`configs`):
+  1  2000^3 NN through cosma::multiply with CosmaMatrix objects (the
+     reference's miniapp shape, cosma_amd/bin/cosma_miniapp) beside the same
+     product by host BLAS;
+  2  the headline GEMM from pinned host memory in 4096^2 tiles, and the
+     layout transforms of a 16384^2 pdgemm on the device;
+  3  tall-skinny A^T B, m = n = 2048, k = 2^20, streamed from pinned host
+     memory in 65536-deep k-panels;
+  4  8192^3 split precision on a = (u - 1/2) exp(phi z), phi in {0, 1, 4},
+     INT8 residues, BF16 residues and slices, against the FP64 DGEMM;
+  5  4096 uneven problems (m, n, k log-uniform in [16, 512]) in one grouped
+     launch replayed as a hipGraph, plain and split precision.
+`--quick` keeps only the headline, the split-precision leg at the headline
+shape and the CPU baseline; `--workload X` runs one leg alone.
 """
 import argparse
+import gc
 import json
 import os
 import re
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_FP64_MFMA = 78.6  # TFLOP/s, MI355X datasheet dense FP64 matrix
+PEAK_FP64_MFMA = 78.6   # TFLOP/s, MI355X datasheet dense FP64 matrix
+PEAK_INT8 = 5000.0      # TOP/s, MI355X datasheet dense INT8 matrix
+PEAK_BF16 = 2500.0      # TFLOP/s, MI355X datasheet dense BF16 matrix
+PEAK_HBM = 8000.0       # GB/s, MI355X datasheet
 M = N = K = 16384
 ALPHA, BETA = 1.0, 0.5
 METRIC = "FP64-accurate TFLOP/s, 1 GPU: native DGEMM vs split-precision at stated error"
+LEGS = ["device", "host", "tall-skinny", "ozaki", "ozaki-target", "reorder", "batched", "miniapp"]
 
 
 def parse():
@@ -32,25 +52,24 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", default="device",
-                   choices=["device", "host", "tall-skinny", "ozaki", "ozaki-target", "reorder", "batched"],
-                   help="device: 16384^3 TN on device-resident operands (default, the headline); "
-                        "host: same GEMM on pinned host operands streamed in 4096^2 tiles; "
-                        "tall-skinny: m=n=2048, k=1048576 A^T B streamed from pinned host memory "
-                        "in k-panels of 65536; ozaki: 8192^3 split-precision vs plain FP64 for "
-                        "phi in {0, 1, 4} plus the 16384^3 A^T B comparison; ozaki-target: only "
-                        "that 16384^3 comparison; reorder: the layout transforms of a 16384^2 pdgemm "
-                        "(A transposed, B copied, C <- alpha X + beta C) on the device; "
-                        "batched: 4096 NN problems with m, n, k log-uniform in [16, 512] in one "
-                        "grouped launch replayed as a hipGraph")
+    p.add_argument("--workload", default=None, choices=LEGS,
+                   help="run one leg alone and print its line: device (the headline with the split-precision "
+                        "leg and the CPU baseline), host (16384^3 from pinned host memory), tall-skinny, ozaki "
+                        "(8192^3 phi sweep with the error of every count), ozaki-target (split precision at the "
+                        "headline shape), reorder, batched, miniapp (2000^3 through cosma::multiply)")
+    p.add_argument("--quick", action="store_true", help="headline, split precision and CPU baseline only")
     p.add_argument("--no-split", action="store_true", help="skip the split-precision summary leg")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
-    p.add_argument("--cpu-n", type=int, default=8192, help="CPU sample size (m = n = k)")
-    p.add_argument("--no-check", action="store_true", help="skip the sampled-row error check")
+    p.add_argument("--cpu-n", type=int, default=16384, help="CPU baseline size (m = n = k)")
+    p.add_argument("--no-check", action="store_true", help="skip the sampled-row error checks")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also run a 2 GiB device copy (block_transform 'N', 2 GiB read + 2 GiB "
                         "written) to calibrate HBM byte counters")
     return p.parse_args()
+
+
+def say(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def traffic_from_profile():
@@ -64,45 +83,51 @@ def traffic_from_profile():
     return float(m.group(1)) if m else None
 
 
+def host_threads():
+    """Host threads for the CPU legs: the machine's share for one GPU (16),
+    or fewer if the process may use fewer cores."""
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
 def cpu_baseline(n):
-    """The reference CPU path (one host-BLAS dgemm, op(A) = A^T, beta = 0.5) on n^3."""
-    import numpy as np
+    """The reference CPU path (one host-BLAS dgemm, op(A) = A^T, beta = 0.5) at
+    the headline shape: one warm-up at n / 2, then two timed calls at n."""
     import oracle
-    threads = min(16, len(os.sched_getaffinity(0)))
+    threads = host_threads()
     oracle.hostblas.set_threads(threads)
+    say(f"cpu baseline: cblas_dgemm TN {n}^3 on {threads} threads")
+    w = n // 2
+    A, B, C = oracle.uniform(w, w, 1), oracle.uniform(w, w, 2), oracle.uniform(w, w, 3)
+    oracle.hostblas.dgemm("T", "N", w, w, w, ALPHA, A, w, B, w, BETA, C, w)   # warm-up (threads, pages)
+    del A, B, C
     A, B, C = oracle.uniform(n, n, 1), oracle.uniform(n, n, 2), oracle.uniform(n, n, 3)
-    oracle.hostblas.dgemm("T", "N", n, n, n, ALPHA, A, n, B, n, BETA, C, n)   # warm-up
     times = []
-    budget = time.perf_counter() + 25.0
-    for _ in range(3):
+    for _ in range(2):
         t0 = time.perf_counter()
         oracle.hostblas.dgemm("T", "N", n, n, n, ALPHA, A, n, B, n, BETA, C, n)
         times.append(time.perf_counter() - t0)
-        if time.perf_counter() > budget:
-            break
     best = min(times)
-    return {"value": round(2.0 * n ** 3 / best / 1e12, 4), "unit": "TFLOP/s", "cores": threads,
-            "kind": "port",
-            "sample": f"cblas_dgemm('T','N') m=n=k={n}, alpha=1, beta=0.5, uniform[-1,1) inputs, "
-                      f"min of {len(times)} reps after 1 warm-up, BLAS {oracle.hostblas.vendor()}"}
+    return {"value": round(2.0 * n ** 3 / best / 1e12, 4), "unit": "TFLOP/s", "cores": threads, "kind": "port",
+            "nproc": os.cpu_count(), "affinity_cores": len(os.sched_getaffinity(0)),
+            "sample": f"cblas_dgemm('T','N') m=n=k={n} (the headline shape), alpha=1, beta=0.5, uniform[-1,1) "
+                      f"inputs, min of {len(times)} calls after a warm-up at {w}^3, BLAS "
+                      f"{oracle.hostblas.vendor()} on {threads} threads (the host share of one GPU on this "
+                      f"machine; OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS', 'unset')})"}
 
 
 def sampled_error(A, B, C, C0_seed, rows=(0, 8191)):
-    """Relative error of sampled rows of C against the double-double oracle."""
+    """Relative error of sampled rows of C (headline product) against the double-double oracle."""
     import numpy as np
     import oracle
-    Bh = np.asfortranarray(B.cpu().numpy().T)
-    num = den = 0.0
+    from oracle.sampled import dd_lines, lines_error
+    Ah, Bh = A.cpu().numpy(), B.cpu().numpy()           # tensors (m, k) / (n, k): A stored k x m
+    r, _ = dd_lines("T", "N", M, N, K, Ah.ctypes.data, K, Bh.ctypes.data, K, rows, ())
+    got, ref = {}, {}
     for i in rows:
-        a_row = np.asfortranarray(A[i].cpu().numpy().reshape(K, 1))
-        ref = np.zeros((1, N), order="F")
-        oracle.dgemm("T", "N", 1, N, K, ALPHA, a_row, K, Bh, K, 0.0, ref, 1, dd=True)
         c0 = oracle.uniform_at(C0_seed, i + np.arange(N, dtype=np.uint64) * M)
-        expect = ref[0] + BETA * c0
-        got = C[:, i].cpu().numpy()
-        num += float(((got - expect) ** 2).sum())
-        den += float((expect ** 2).sum())
-    return (num / den) ** 0.5
+        ref[i] = ALPHA * r[i] + BETA * c0
+        got[i] = C[:, i].cpu().numpy()
+    return lines_error(got, {}, ref, {})
 
 
 def link_bandwidth(torch, nbytes=1 << 31):
@@ -118,21 +143,27 @@ def link_bandwidth(torch, nbytes=1 << 31):
     return 3 * nbytes / (time.perf_counter() - t0) / 1e9
 
 
+def fill_pinned(t, seed):
+    """Uniform fill of a pinned host tensor (cols, rows), generated on the GPU in pieces."""
+    import torch
+    import cosma_amd as ca
+    cols, rows = t.shape
+    step = max(1, (1 << 28) // rows)
+    tmp = torch.empty((min(step, cols), rows), dtype=torch.float64, device="cuda")
+    for c0 in range(0, cols, step):
+        cc = min(step, cols - c0)
+        ca.fill_uniform(tmp.data_ptr(), rows, cc, rows, seed, offset=c0 * rows)
+        t[c0:c0 + cc].copy_(tmp[:cc])
+    torch.cuda.synchronize()
+    del tmp
+
+
 def streamed_error(h, ta, tb, m, n, k, alpha, beta, host, C0_seed):
     """One more call from the seeded C0, then the relative error of rows 0 and
     m-1 and column n/2 of C against the double-double oracle (all k terms)."""
-    import torch
-    import cosma_amd as ca
     from oracle.sampled import dd_lines, lines_error
     C = host["C"]                                   # torch (n, m): row j = column j of C
-    cols_dev = max(1, (1 << 28) // m)
-    tmp = torch.empty((min(cols_dev, n), m), dtype=torch.float64, device="cuda")
-    for c0 in range(0, n, cols_dev):
-        cc = min(cols_dev, n - c0)
-        ca.fill_uniform(tmp.data_ptr(), m, cc, m, C0_seed, offset=c0 * m)
-        C[c0:c0 + cc].copy_(tmp[:cc])
-    torch.cuda.synchronize()
-    del tmp
+    fill_pinned(C, C0_seed)
     rows, cols = (0, m - 1), (n // 2,)
     c0r = {i: C[:, i].numpy().copy() for i in rows}
     c0c = {j: C[j, :].numpy().copy() for j in cols}
@@ -143,11 +174,11 @@ def streamed_error(h, ta, tb, m, n, k, alpha, beta, host, C0_seed):
                        {j: alpha * c[j] + beta * c0c[j] for j in cols})
 
 
-def streamed(args):
+def streamed(kind, steps, warmup, check=True):
     """Host-resident workloads through the host-streaming DGEMM."""
     import torch
     import cosma_amd as ca
-    if args.workload == "host":
+    if kind == "host":
         m = n = k = 16384
         ta, tb, alpha, beta = "T", "N", 1.0, 0.5
         tiles = (4096, 4096, 4096)
@@ -158,20 +189,12 @@ def streamed(args):
         ta, tb, alpha, beta = "T", "N", 1.0, 1.0
         tiles = (4096, 4096, 65536)
         desc = "dgemm op(A)=A^T m=n=2048 k=1048576 alpha=beta=1, pinned host operands, k-panels of 65536"
+    say(f"streamed leg: {desc}")
     h = ca.Handle(0, streams=2, tile_m=tiles[0], tile_n=tiles[1], tile_k=tiles[2])
-    # pinned host operands (column-major; A stored k x m), filled on the GPU and copied down
-    shapes = {"A": (m, k), "B": (n, k), "C": (n, m)}            # torch (cols, rows)
+    shapes = {"A": (m, k), "B": (n, k), "C": (n, m)}            # torch (cols, rows); A stored k x m
     host = {x: torch.empty(sh, dtype=torch.float64, pin_memory=True) for x, sh in shapes.items()}
-    for seed, (x, sh) in enumerate(shapes.items(), start=11):
-        cols = sh[0]
-        step_cols = max(1, (1 << 28) // sh[1])
-        tmp = torch.empty((min(step_cols, cols), sh[1]), dtype=torch.float64, device="cuda")
-        for c0 in range(0, cols, step_cols):
-            cc = min(step_cols, cols - c0)
-            ca.fill_uniform(tmp.data_ptr(), sh[1], cc, sh[1], seed, offset=c0 * sh[1])
-            host[x][c0:c0 + cc].copy_(tmp[:cc])
-        del tmp
-    torch.cuda.synchronize()
+    for seed, x in enumerate(shapes, start=11):
+        fill_pinned(host[x], seed)
     C0_seed = 13
     link = link_bandwidth(torch)
 
@@ -180,26 +203,25 @@ def streamed(args):
                      host["C"].data_ptr(), m)
 
     times = []
-    for i in range(args.warmup + args.steps):
+    for i in range(warmup + steps):
         t0 = time.perf_counter()
         call()
         dt = time.perf_counter() - t0
-        if i >= args.warmup:
+        if i >= warmup:
             times.append(dt)
     t = sum(times) / len(times)
-    err = None if args.no_check else streamed_error(h, ta, tb, m, n, k, alpha, beta, host, C0_seed)
+    mode = h.last_mode()
+    err = streamed_error(h, ta, tb, m, n, k, alpha, beta, host, C0_seed) if check else None
     flops = 2.0 * m * n * k
     h2d = 8.0 * (m * k + k * n + (m * n if beta != 0 else 0))
     d2h = 8.0 * m * n
     t_mfma = flops / (PEAK_FP64_MFMA * 1e12)
     t_link = h2d / (link * 1e9)
     out = {
-        "metric": METRIC, "value": round(flops / t / 1e12, 3), "unit": "TFLOP/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t * 1e3, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic (seeded SplitMix64 uniform[-1,1))",
+        "value": round(flops / t / 1e12, 3), "unit": "TFLOP/s", "steps": steps, "warmup": warmup,
+        "ms_per_step": round(t * 1e3, 3),
         "config": {"workload": desc, "m": m, "n": n, "k": k, "tiles": tiles, "schedule":
-                   {0: "device", 1: "resident panels", 2: "bounded ring"}[h.last_mode()]},
+                   {0: "device", 1: "resident panels", 2: "bounded ring"}[mode]},
         "error": {"rel_fro_sampled_lines_vs_double_double": err, "bound_4k_u": 4 * k * 2.0 ** -53,
                   "lines": "rows 0, m-1 and column n/2 of C, each with all k terms"},
         "roofline": {"bound": "host-link" if t_link > t_mfma else "mfma",
@@ -208,11 +230,31 @@ def streamed(args):
                      "note": "achieved = host->device bytes / time; peak = measured pinned H2D copy rate; "
                              "frac = max(link time, MFMA time) / time"},
     }
-    print(json.dumps(out), flush=True)
+    h.close()
+    del host
+    gc.collect()
+    torch.cuda.empty_cache()
+    return out
 
 
-def rows_error(Ad, Bd, Cd, n, rows, ref_rows):
-    """Relative Frobenius error of sampled rows of C (device, column-major n x n)."""
+def dd_rows_nn(Ad, Bd, n, rows):
+    """Rows of A*B (NN, n x n column-major device operands) from the double-double oracle."""
+    from oracle.sampled import dd_lines
+    Ah, Bh = Ad.cpu().numpy(), Bd.cpu().numpy()
+    r, _ = dd_lines("N", "N", n, n, n, Ah.ctypes.data, n, Bh.ctypes.data, n, rows, ())
+    return [r[i] for i in rows]
+
+
+def dd_rows_tn(Ad, Bd, n, rows):
+    """Rows of A^T*B (A stored k x m, B k x n, column-major device operands)."""
+    from oracle.sampled import dd_lines
+    Ah, Bh = Ad.cpu().numpy(), Bd.cpu().numpy()
+    r, _ = dd_lines("T", "N", n, n, n, Ah.ctypes.data, n, Bh.ctypes.data, n, rows, ())
+    return [r[i] for i in rows]
+
+
+def rows_error(Cd, rows, ref_rows):
+    """Relative Frobenius error of sampled rows of C (device, column-major)."""
     num = den = 0.0
     for i, ref in zip(rows, ref_rows):
         got = Cd[:, i].cpu().numpy()
@@ -221,27 +263,41 @@ def rows_error(Ad, Bd, Cd, n, rows, ref_rows):
     return (num / den) ** 0.5
 
 
-def dd_rows(Ad, Bd, n, rows):
-    """Rows of A*B (NN, n x n column-major device operands) from the double-double oracle."""
-    import numpy as np
-    import oracle
-    Bh = np.asfortranarray(Bd.cpu().numpy().T)
-    Ah = Ad.cpu().numpy()                 # tensor (k, m): row l = column l of A
-    out = []
-    for i in rows:
-        a_row = np.asfortranarray(Ah[:, i].reshape(1, n))
-        ref = np.zeros((1, n), order="F")
-        oracle.dgemm("N", "N", 1, n, n, 1.0, a_row, 1, Bh, n, 0.0, ref, 1, dd=True)
-        out.append(ref[0])
-    return out
+def timed(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
 
 
-def split_precision(phis=(0.0, 1.0, 4.0), steps=5, warmup=1, n=8192, rows=(0, 4097)):
+def ozaki_entry(info, t, t_plain, flops, err, n):
+    """Per-method record of a split-precision leg, with the product kernel's
+    roofline (INT8 or BF16 matrix peak; 2 n^3 ops per product)."""
+    bf16 = info["method"] == 2
+    ops = 2.0 * n ** 3 * info["int8_products"]
+    prod = ops / (info["product_ms"] * 1e-3) / 1e12 if info["product_ms"] > 0 else None
+    peak = PEAK_BF16 if bf16 else PEAK_INT8
+    return {"tflops_fp64_equiv": round(flops / t / 1e12, 3), "ms": round(t * 1e3, 3),
+            "speedup_vs_fp64": round(t_plain / t, 3), "rel_err": err, "count": info["count"], "bits": info["bits"],
+            "products": info["int8_products"], "error_estimate": info["error_estimate"],
+            "error_bound": info["error_bound"], "tolerance_met": info["tolerance_met"], "passes": info["passes"],
+            "roofline": {"bound": "mfma", "kernel": "i8gemm_kernel (all products of the call)",
+                         "achieved": round(prod, 1) if prod else None, "peak": peak,
+                         "unit": "TFLOP/s" if bf16 else "TOP/s", "frac": round(prod / peak, 4) if prod else None,
+                         "traffic": None}}
+
+
+def split_precision(phis=(0.0, 1.0, 4.0), steps=5, warmup=1, n=8192, rows=(0, 4097), sweep=True):
     """8192^3 C = A*B (NN): plain FP64 MFMA DGEMM vs the split-precision DGEMM
-    (moduli at the default 1e-14 tolerance, with INT8 and with BF16 residues;
-    slices with 7 slices), inputs
-    a = (u - 0.5) exp(phi z).  Errors are relative Frobenius errors of sampled
-    rows against the double-double oracle."""
+    (moduli at the default 1e-14 tolerance with INT8 and with BF16 residues,
+    slices at the default tolerance), inputs a = (u - 0.5) exp(phi z).  Errors
+    are relative Frobenius errors of sampled rows against the double-double
+    oracle.  With `sweep`, also the error and time of every count."""
     import torch
     import cosma_amd as ca
     h = ca.Handle(0)
@@ -251,78 +307,46 @@ def split_precision(phis=(0.0, 1.0, 4.0), steps=5, warmup=1, n=8192, rows=(0, 40
     flops = 2.0 * n ** 3
     results = []
     for phi in phis:
+        say(f"split precision 8192^3 phi={phi}")
         ca.fill_lognormal(A.data_ptr(), n, n, n, 21, phi)
         ca.fill_lognormal(B.data_ptr(), n, n, n, 22, phi)
         torch.cuda.synchronize()
-        ref = dd_rows(A, B, n, rows)
-
-        def timed(fn):
-            for _ in range(warmup):
-                fn()
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(steps):
-                fn()
-            torch.cuda.synchronize()
-            return (time.perf_counter() - t0) / steps
-
+        ref = dd_rows_nn(A, B, n, rows)
         plain = lambda: h.dgemm("N", "N", n, n, n, 1.0, A.data_ptr(), n, B.data_ptr(), n, 0.0, C.data_ptr(), n)
-        t_plain = timed(plain)
-        e_plain = rows_error(A, B, C, n, rows, ref)
+        t_plain = timed(plain, steps, warmup)
         entry = {"phi": phi, "fp64": {"tflops": round(flops / t_plain / 1e12, 3), "ms": round(t_plain * 1e3, 3),
-                                      "rel_err": e_plain}}
-        for name, method, count in (("moduli", ca.OZAKI_MODULI, 0), ("moduli_bf16", ca.OZAKI_MODULI_BF16, 0),
-                                    ("slices", ca.OZAKI_SLICES, 7)):
+                                      "rel_err": rows_error(C, rows, ref)}}
+        for name, method in (("moduli", ca.OZAKI_MODULI), ("moduli_bf16", ca.OZAKI_MODULI_BF16),
+                             ("slices", ca.OZAKI_SLICES)):
             info = {}
 
             def oz():
                 info.update(h.dgemm_ozaki("N", "N", n, n, n, 1.0, A.data_ptr(), n, B.data_ptr(), n, 0.0,
-                                          C.data_ptr(), n, method=method, tolerance=1e-14, count=count))
-            t = timed(oz)
-            entry[name] = {"tflops_fp64_equiv": round(flops / t / 1e12, 3), "ms": round(t * 1e3, 3),
-                           "speedup_vs_fp64": round(t_plain / t, 3), "rel_err": rows_error(A, B, C, n, rows, ref),
-                           "count": info["count"], "bits": info["bits"], "int8_products": info["int8_products"],
-                           "error_estimate": info["error_bound"]}
-        # error against the double-double oracle for each count: moduli 8..20
-        # and slices 4..8 (one warm-up and one timed call each)
-        sweep = []
-        for name, method, counts in (("moduli", ca.OZAKI_MODULI, range(8, 21, 2)),
-                                     ("slices", ca.OZAKI_SLICES, range(4, 9))):
-            for cnt in counts:
-                info = {}
+                                          C.data_ptr(), n, method=method, tolerance=1e-14))
+            t = timed(oz, steps, warmup)
+            entry[name] = ozaki_entry(info, t, t_plain, flops, rows_error(C, rows, ref), n)
+        if sweep:
+            # error against the double-double oracle for each count: moduli 8..20
+            # and slices 4..8 (one warm-up and one timed call each)
+            sw = []
+            for name, method, counts in (("moduli", ca.OZAKI_MODULI, range(8, 21, 2)),
+                                         ("slices", ca.OZAKI_SLICES, range(4, 9))):
+                for cnt in counts:
+                    info = {}
 
-                def oz():
-                    info.update(h.dgemm_ozaki("N", "N", n, n, n, 1.0, A.data_ptr(), n, B.data_ptr(), n, 0.0,
-                                              C.data_ptr(), n, method=method, count=cnt))
-                oz()
-                torch.cuda.synchronize()
-                t0 = time.perf_counter()
-                oz()
-                torch.cuda.synchronize()
-                t = time.perf_counter() - t0
-                sweep.append({"method": name, "count": cnt, "bits": info["bits"], "ms": round(t * 1e3, 3),
-                              "rel_err": rows_error(A, B, C, n, rows, ref), "error_estimate": info["error_bound"]})
-        entry["count_sweep"] = sweep
+                    def oz():
+                        info.update(h.dgemm_ozaki("N", "N", n, n, n, 1.0, A.data_ptr(), n, B.data_ptr(), n, 0.0,
+                                                  C.data_ptr(), n, method=method, count=cnt))
+                    t = timed(oz, 1, 1)
+                    sw.append({"method": name, "count": cnt, "bits": info["bits"], "ms": round(t * 1e3, 3),
+                               "rel_err": rows_error(C, rows, ref), "error_estimate": info["error_estimate"],
+                               "error_bound": info["error_bound"]})
+            entry["count_sweep"] = sw
         results.append(entry)
+    h.close()
     del A, B, C
     torch.cuda.empty_cache()
     return results
-
-
-def dd_rows_tn(Ad, Bd, n, rows):
-    """Rows of A^T*B (A stored k x m, B k x n, column-major device operands)
-    from the double-double oracle."""
-    import numpy as np
-    import oracle
-    Bh = np.asfortranarray(Bd.cpu().numpy().T)
-    Ah = Ad.cpu().numpy()                 # tensor (m, k): row i = column i of stored A = row i of A^T
-    out = []
-    for i in rows:
-        a_row = np.asfortranarray(Ah[i, :].reshape(1, n))
-        ref = np.zeros((1, n), order="F")
-        oracle.dgemm("N", "N", 1, n, n, 1.0, a_row, 1, Bh, n, 0.0, ref, 1, dd=True)
-        out.append(ref[0])
-    return out
 
 
 def split_precision_target(steps=3, warmup=1, n=16384, rows=(0, 8193)):
@@ -333,6 +357,7 @@ def split_precision_target(steps=3, warmup=1, n=16384, rows=(0, 8193)):
     oracle."""
     import torch
     import cosma_amd as ca
+    say("split precision at the headline shape")
     h = ca.Handle(0)
     A = torch.empty((n, n), dtype=torch.float64, device="cuda")
     B = torch.empty((n, n), dtype=torch.float64, device="cuda")
@@ -342,45 +367,33 @@ def split_precision_target(steps=3, warmup=1, n=16384, rows=(0, 8193)):
     torch.cuda.synchronize()
     ref = dd_rows_tn(A, B, n, rows)
     flops = 2.0 * n ** 3
-
-    def timed(fn):
-        for _ in range(warmup):
-            fn()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            fn()
-        torch.cuda.synchronize()
-        return (time.perf_counter() - t0) / steps
-
-    t_plain = timed(lambda: h.dgemm("T", "N", n, n, n, 1.0, A.data_ptr(), n, B.data_ptr(), n, 0.0, C.data_ptr(), n))
-    e_plain = rows_error(A, B, C, n, rows, ref)
+    t_plain = timed(lambda: h.dgemm("T", "N", n, n, n, 1.0, A.data_ptr(), n, B.data_ptr(), n, 0.0, C.data_ptr(), n),
+                    steps, warmup)
+    e_plain = rows_error(C, rows, ref)
     info = {}
 
     def oz():
         info.update(h.dgemm_ozaki("T", "N", n, n, n, 1.0, A.data_ptr(), n, B.data_ptr(), n, 0.0, C.data_ptr(), n,
                                   method=ca.OZAKI_MODULI, tolerance=1e-14))
-    t_oz = timed(oz)
+    t_oz = timed(oz, steps, warmup)
     out = {"workload": "C=A^T*B m=n=k=16384 alpha=1 beta=0, uniform[-1,1) inputs",
            "fp64": {"tflops": round(flops / t_plain / 1e12, 3), "ms": round(t_plain * 1e3, 3), "rel_err": e_plain},
-           "moduli": {"tflops_fp64_equiv": round(flops / t_oz / 1e12, 3), "ms": round(t_oz * 1e3, 3),
-                      "speedup_vs_fp64": round(t_plain / t_oz, 3), "rel_err": rows_error(A, B, C, n, rows, ref),
-                      "count": info["count"], "bits": info["bits"], "int8_products": info["int8_products"],
-                      "error_estimate": info["error_bound"]}}
+           "moduli": ozaki_entry(info, t_oz, t_plain, flops, rows_error(C, rows, ref), n)}
+    h.close()
     del A, B, C
     torch.cuda.empty_cache()
     return out
 
 
-PEAK_HBM = 8000.0  # GB/s, MI355X datasheet
-
-
-def reorder(args):
+def reorder(steps, warmup):
     """Device block transforms of the ScaLAPACK -> COSMA -> ScaLAPACK round trip
     of a 16384^2 pdgemm on one rank: A transposed (op = 'T'), B copied
-    (op = 'N'), the result written back as C <- alpha * X + beta * C."""
+    (op = 'N'), the result written back as C <- alpha * X + beta * C; the
+    transposed copy is checked bit for bit on sampled columns."""
+    import numpy as np
     import torch
     import cosma_amd as ca
+    say("reorder leg: 16384^2 transforms")
     n = 16384
     h = ca.Handle(0)
     X = torch.empty((n, n), dtype=torch.float64, device="cuda")
@@ -398,30 +411,34 @@ def reorder(args):
             "C epilogue (alpha=1, beta=0.5)": (lambda: h.block_transform("N", n, n, 1.0, X.data_ptr(), n, 0.5,
                                                                            Z.data_ptr(), n), 3 * nb)}
     res, total_b, total_t = {}, 0.0, 0.0
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     for name, (fn, nbytes) in legs.items():
-        for _ in range(args.warmup):
+        for _ in range(warmup):
             fn()
-        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
         s.record()
-        for _ in range(args.steps):
+        for _ in range(steps):
             fn()
         e.record()
         torch.cuda.synchronize()
-        ms = s.elapsed_time(e) / args.steps
+        ms = s.elapsed_time(e) / steps
         res[name] = {"ms": round(ms, 4), "GB/s": round(nbytes / ms / 1e6, 1),
                      "frac_of_hbm": round(nbytes / ms / 1e6 / PEAK_HBM, 4)}
         total_b += nbytes
         total_t += ms
-    out = {"metric": "device layout-transform bandwidth (algorithmic bytes / time)",
-           "value": round(total_b / total_t / 1e6, 1), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
-           "warmup": args.warmup, "ms_per_step": round(total_t, 4), "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded uniform)",
-           "config": {"workload": "16384^2 ScaLAPACK <-> COSMA transforms of one pdgemm on one rank"},
-           "legs": res,
-           "roofline": {"bound": "hbm", "achieved": round(total_b / total_t / 1e6, 1), "peak": PEAK_HBM,
-                        "unit": "GB/s", "frac": round(total_b / total_t / 1e6 / PEAK_HBM, 4), "traffic": None}}
-    print(json.dumps(out), flush=True)
+    # bit-exactness of the transpose on sampled columns of the result
+    h.block_transform("T", n, n, 1.0, X.data_ptr(), n, 0.0, Y.data_ptr(), n)
+    torch.cuda.synchronize()
+    exact = all(np.array_equal(Y[j].cpu().numpy(), X[:, j].cpu().numpy()) for j in (0, 4097, n - 1))
+    h.close()
+    del X, Y, Z
+    torch.cuda.empty_cache()
+    return {"value": round(total_b / total_t / 1e6, 1), "unit": "GB/s", "steps": steps, "warmup": warmup,
+            "ms_per_step": round(total_t, 4),
+            "config": {"workload": "16384^2 ScaLAPACK <-> COSMA transforms of one pdgemm on one rank"},
+            "legs": res, "bit_exact_transpose_sampled_columns": exact,
+            "roofline": {"bound": "hbm", "achieved": round(total_b / total_t / 1e6, 1), "peak": PEAK_HBM,
+                         "unit": "GB/s", "frac": round(total_b / total_t / 1e6 / PEAK_HBM, 4), "traffic": None}}
 
 
 def batch_dims(count=4096, seed=5):
@@ -432,15 +449,16 @@ def batch_dims(count=4096, seed=5):
     return np.clip(np.rint(np.exp(rng.uniform(np.log(16), np.log(512), (count, 3)))), 16, 512).astype(np.int64)
 
 
-def batched(args, count=4096, seed=5):
-    """4096 independent C = A B problems (NN, alpha = 1, beta = 0), m, n, k each
-    round(exp(U[ln 16, ln 512])) clamped to [16, 512] (numpy default_rng(5)),
-    operands packed in three device buffers; one grouped persistent launch per
-    step, replayed as a hipGraph after the plan is tuned; then the same batch
-    in split-precision form (grouped moduli plan at the 1e-14 tolerance)."""
+def batched(steps, warmup, count=4096, seed=5):
+    """4096 independent C = A B problems (NN, alpha = 1, beta = 0) packed in
+    three device buffers; one grouped persistent launch per step, replayed as
+    a hipGraph after the plan is tuned; then the same batch in split-precision
+    form (grouped moduli plan at the 1e-14 tolerance) with its check."""
     import numpy as np
     import torch
     import cosma_amd as ca
+    import oracle
+    say("batched leg: 4096 problems")
     dims = batch_dims(count, seed)
 
     def offsets(sizes):
@@ -460,130 +478,145 @@ def batched(args, count=4096, seed=5):
     h = ca.Handle(0)
     plan = ca.Grouped(h, probs)
     plan.tune()
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         plan.graph_run()
     torch.cuda.synchronize()
     s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     s.record()
-    for _ in range(args.steps):
+    for _ in range(steps):
         plan.graph_run()
     e.record()
     torch.cuda.synchronize()
-    wall = (time.perf_counter() - t0) / args.steps
-    ms = s.elapsed_time(e) / args.steps
+    wall = (time.perf_counter() - t0) / steps
+    ms = s.elapsed_time(e) / steps
     flops = 2.0 * (m * n * k).astype(np.float64)
     nbytes = 8.0 * (m * k + k * n + m * n).astype(np.float64)
     t_roof = float(np.maximum(flops / (PEAK_FP64_MFMA * 1e12), nbytes / (PEAK_HBM * 1e9)).sum())
     # correctness of a sample of problems against the double-double oracle
-    import oracle
+    sample = sorted(set([0, 1, count // 2, count - 1] + np.argsort(m * n * k)[-4:].tolist()))
+    Ah, Bh = A.cpu().numpy(), B.cpu().numpy()
     refs = {}
+    for i in sample:
+        ref = np.zeros((int(m[i]), int(n[i])), order="F")
+        oracle.dgemm_raw("N", "N", int(m[i]), int(n[i]), int(k[i]), 1.0, Ah.ctypes.data + 8 * int(oa[i]), int(m[i]),
+                         Bh.ctypes.data + 8 * int(ob[i]), int(k[i]), 0.0, ref.ctypes.data, int(m[i]), dd=True)
+        refs[i] = ref
 
-    def sampled_error():
-        errs = []
-        for i in (0, 1, count // 2, count - 1):
-            Ch = C[int(oc[i]):int(oc[i]) + int(m[i] * n[i])].cpu().numpy().reshape((int(n[i]), int(m[i]))).T
-            if i not in refs:
-                Ah = A[int(oa[i]):int(oa[i]) + int(m[i] * k[i])].cpu().numpy().reshape((int(k[i]), int(m[i]))).T
-                Bh = B[int(ob[i]):int(ob[i]) + int(k[i] * n[i])].cpu().numpy().reshape((int(n[i]), int(k[i]))).T
-                ref = np.zeros((int(m[i]), int(n[i])), order="F")
-                oracle.dgemm("N", "N", int(m[i]), int(n[i]), int(k[i]), 1.0, np.asfortranarray(Ah), int(m[i]),
-                             np.asfortranarray(Bh), int(k[i]), 0.0, ref, int(m[i]), dd=True)
-                refs[i] = ref
-            errs.append(oracle.rel_fro(Ch, refs[i]))
-        return max(errs)
+    def sampled_err():
+        Ch = C.cpu().numpy()
+        return max(oracle.rel_fro(Ch[int(oc[i]):int(oc[i]) + int(m[i] * n[i])].reshape((int(n[i]), int(m[i]))).T,
+                                  refs[i]) for i in sample)
 
-    errs = [sampled_error()]
+    err = sampled_err()
     info = plan.info()
     value = float(flops.sum()) / wall / 1e12
-    out = {"metric": METRIC, "value": round(value, 3), "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps,
-           "warmup": args.warmup, "ms_per_step": round(wall * 1e3, 4), "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-           "data": "synthetic (seeded uniform operands, seeded log-uniform sizes)",
+    out = {"value": round(value, 3), "unit": "TFLOP/s", "steps": steps, "warmup": warmup,
+           "ms_per_step": round(wall * 1e3, 4),
            "config": {"workload": f"{count} NN problems, m,n,k log-uniform in [16,512], one grouped launch per "
                                   f"step replayed as a hipGraph", "sum_flops": float(flops.sum()),
                       "sum_bytes": float(nbytes.sum()), "plan": info},
-           "error": {"max_rel_fro_sampled_problems_vs_double_double": max(errs)},
-           "roofline": {"bound": "per-problem max(MFMA, HBM)", "achieved": round(value, 3),
+           "error": {"max_rel_fro_sampled_problems_vs_double_double": err, "sampled_problems": len(sample)},
+           "roofline": {"bound": "per-problem max(MFMA, HBM)",
+                        "achieved": round(float(flops.sum()) / (ms * 1e-3) / 1e12, 3),
                         "peak": round(float(flops.sum()) / t_roof / 1e12, 3), "unit": "TFLOP/s",
                         "frac": round(t_roof / (ms * 1e-3), 4), "traffic": None,
                         "note": "peak = sum flops / sum_i max(flops_i / 78.6 TF, bytes_i / 8 TB/s); "
                                 "frac uses the launch-inclusive event time of a graph replay"}}
     plan.close()
-    # the split-precision form of the same batch: one plan at the default 1e-14
-    # tolerance (moduli chosen from these operands), tuned, replayed as a graph
     oz = ca.GroupedOzaki(h, probs)
     oz.tune()
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         oz.graph_run()
     torch.cuda.synchronize()
     s.record()
-    for _ in range(args.steps):
+    for _ in range(steps):
         oz.graph_run()
     e.record()
     torch.cuda.synchronize()
-    oz_ms = s.elapsed_time(e) / args.steps
+    oz_ms = s.elapsed_time(e) / steps
     out["ozaki_form"] = {"ms_per_step": round(oz_ms, 4),
                          "tflops_fp64_equiv": round(float(flops.sum()) / (oz_ms * 1e-3) / 1e12, 3),
-                         "vs_plain_fp64": round(ms / oz_ms, 3), "plan": oz.info(),
-                         "max_rel_fro_sampled_problems_vs_double_double": sampled_error()}
+                         "vs_plain_fp64": round(ms / oz_ms, 3), "plan": oz.info(), "check": oz.check(1e-14),
+                         "max_rel_fro_sampled_problems_vs_double_double": sampled_err()}
     oz.close()
-    print(json.dumps(out), flush=True)
+    h.close()
+    del A, B, C
+    torch.cuda.empty_cache()
+    return out
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path",
-              file=sys.stderr)
-        sys.exit(2)
-    if args.workload == "ozaki-target":
-        target = split_precision_target(steps=args.steps, warmup=args.warmup)
-        out = {"metric": METRIC, "value": target["moduli"]["tflops_fp64_equiv"], "unit": "TFLOP/s", "n_gpus": 1,
-               "steps": args.steps, "warmup": args.warmup, "ms_per_step": target["moduli"]["ms"],
-               "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64 (int8 MFMA inside)",
-               "data": "synthetic (seeded SplitMix64 uniform[-1,1), generated in HBM)",
-               "config": {"workload": target["workload"]}, "result": target}
-        print(json.dumps(out), flush=True)
-        return
-    if args.workload == "ozaki":
-        res = split_precision(steps=args.steps, warmup=args.warmup)
-        target = split_precision_target(steps=min(args.steps, 3), warmup=1)
-        best = max(r["moduli"]["tflops_fp64_equiv"] for r in res)
-        out = {"metric": METRIC, "value": best, "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps,
-               "warmup": args.warmup, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-               "dtype": "f64 (int8 MFMA inside)", "data": "synthetic a=(u-0.5)exp(phi z), seeded",
-               "config": {"workload": "C=A*B m=n=k=8192, plain FP64 vs split precision, phi in {0,1,4}"},
-               "results": res, "target_shape": target,
-               "error_note": "rel_err = relative Frobenius error of 2 sampled rows vs the double-double oracle"}
-        print(json.dumps(out), flush=True)
-        return
-    if args.workload == "reorder":
-        return reorder(args)
-    if args.workload == "batched":
-        return batched(args)
-    if args.workload != "device":
-        return streamed(args)
+def miniapp(reps=5, n=2000):
+    """The reference's miniapp shape: C = A B, m = n = k = 2000, NN, alpha = 1,
+    beta = 0, through cosma::multiply with CosmaMatrix objects on one rank
+    (cosma_amd/bin/cosma_miniapp: host matrices in the page-locked pool,
+    streamed to the GPU), and the same product by host BLAS (the reference's
+    CPU path), with the error of sampled lines of C."""
+    import numpy as np
+    import oracle
+    from oracle.sampled import dd_lines, lines_error
+    from cosma_amd import build as cbuild
+    say("miniapp leg: 2000^3 through cosma::multiply")
+    exe = cbuild.MINIAPP
+    if not os.path.exists(exe):
+        raise RuntimeError(f"{exe} is not built")
+    with tempfile.TemporaryDirectory() as d:
+        out_path = os.path.join(d, "c.bin")
+        r = subprocess.run([exe, "-m", str(n), "-n", str(n), "-k", str(n), "-r", str(reps), "--json", "--out",
+                            out_path], capture_output=True, text=True, timeout=300)
+        if r.returncode != 0:
+            raise RuntimeError(f"cosma_miniapp failed: {r.stderr}")
+        gpu = json.loads(r.stdout.strip().splitlines()[-1])
+        Cg = np.fromfile(out_path, dtype=np.float64).reshape((n, n), order="F")
+    A, B = oracle.uniform(n, n, 1), oracle.uniform(n, n, 2)
+    rows, cols = (0, n - 1), (n // 2,)
+    rr, cc = dd_lines("N", "N", n, n, n, A.ctypes.data, n, B.ctypes.data, n, rows, cols)
+    err = lines_error({i: Cg[i, :] for i in rows}, {j: Cg[:, j] for j in cols}, rr, cc)
+    threads = host_threads()
+    oracle.hostblas.set_threads(threads)
+    Cc = np.zeros((n, n), order="F")
+    oracle.hostblas.dgemm("N", "N", n, n, n, 1.0, A, n, B, n, 0.0, Cc, n)
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        oracle.hostblas.dgemm("N", "N", n, n, n, 1.0, A, n, B, n, 0.0, Cc, n)
+        times.append(time.perf_counter() - t0)
+    cpu_err = lines_error({i: Cc[i, :] for i in rows}, {j: Cc[:, j] for j in cols}, rr, cc)
+    flops = 2.0 * n ** 3
+    return {"value": round(flops / (gpu["median_ms"] * 1e-3) / 1e12, 3), "unit": "TFLOP/s",
+            "config": {"workload": f"C=A*B m=n=k={n} NN alpha=1 beta=0 through cosma::multiply (CosmaMatrix, one "
+                                   f"rank, no steps), host matrices in the page-locked memory pool"},
+            "gpu": {"ms": gpu["ms"], "median_ms": gpu["median_ms"], "first_call_ms": gpu["ms"][0],
+                    "rel_err_sampled_lines_vs_double_double": err},
+            "cpu_reference_path": {"tflops": round(flops / min(times) / 1e12, 4),
+                                   "min_ms": round(min(times) * 1e3, 3), "threads": threads,
+                                   "blas": oracle.hostblas.vendor(),
+                                   "rel_err_sampled_lines_vs_double_double": cpu_err},
+            "speedup_vs_cpu": round(min(times) * 1e3 / gpu["median_ms"], 2),
+            "roofline": {"bound": "host-link", "traffic": 3 * 8.0 * n * n,
+                         "note": "96 MB of operands cross the host link per call (A, B up, C down); "
+                                 "the 16 GFLOP are 0.2 ms of FP64 MFMA time"}}
+
+
+def headline(args):
+    """The headline DGEMM: its line (without the CPU baseline)."""
     import torch
     import cosma_amd as ca
-
+    say("headline: 16384^3 TN beta=0.5 device-resident")
     dev = torch.device("cuda:0")
     h = ca.Handle(0)
-    # column-major 16384 x 16384 operands; A is stored k x m (op(A) = A^T)
-    A = torch.empty((M, K), dtype=torch.float64, device=dev)
+    A = torch.empty((M, K), dtype=torch.float64, device=dev)     # column-major A stored k x m (op(A) = A^T)
     B = torch.empty((N, K), dtype=torch.float64, device=dev)
     C = torch.empty((N, M), dtype=torch.float64, device=dev)
     ca.fill_uniform(A.data_ptr(), K, M, K, 11)
     ca.fill_uniform(B.data_ptr(), K, N, K, 12)
     ca.fill_uniform(C.data_ptr(), M, N, M, 13)
     torch.cuda.synchronize()
-
     if args.calibrate_copy:
         # 16384 x 16384 doubles = 2 GiB read and 2 GiB written per launch
         h.block_transform("N", M, N, 1.0, B.data_ptr(), K, 0.0, C.data_ptr(), M)
         torch.cuda.synchronize()
         ca.fill_uniform(C.data_ptr(), M, N, M, 13)
-
     # correctness of one call from the seeded C0 (outside the timed region)
     h.dgemm("T", "N", M, N, K, ALPHA, A.data_ptr(), K, B.data_ptr(), K, BETA, C.data_ptr(), M)
     torch.cuda.synchronize()
@@ -610,7 +643,6 @@ def main():
     flops = 2.0 * M * N * K
     value = flops * args.steps / wall / 1e12
     achieved = flops / (kernel_ms * 1e-3) / 1e12
-
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -631,10 +663,68 @@ def main():
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_MFMA, 4),
                      "traffic": traffic_from_profile()},
     }
+    h.close()
     del A, B, C
     torch.cuda.empty_cache()
+    return out
+
+
+def leg_line(res, steps, warmup):
+    """A leg run alone: its record as a full line of the metric."""
+    line = {"metric": METRIC, "n_gpus": 1, "steps": steps, "warmup": warmup, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded)"}
+    line.update(res)
+    return line
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path",
+              file=sys.stderr)
+        sys.exit(2)
+    w = args.workload
+    if w == "ozaki-target":
+        target = split_precision_target(steps=args.steps, warmup=args.warmup)
+        print(json.dumps(leg_line({"value": target["moduli"]["tflops_fp64_equiv"], "unit": "TFLOP/s",
+                                   "ms_per_step": target["moduli"]["ms"], "config": {"workload": target["workload"]},
+                                   "result": target}, args.steps, args.warmup)), flush=True)
+        return
+    if w == "ozaki":
+        res = split_precision(steps=args.steps, warmup=args.warmup)
+        best = max(r["moduli"]["tflops_fp64_equiv"] for r in res)
+        print(json.dumps(leg_line({"value": best, "unit": "TFLOP/s", "config": {
+            "workload": "C=A*B m=n=k=8192, plain FP64 vs split precision, phi in {0,1,4}"}, "results": res},
+            args.steps, args.warmup)), flush=True)
+        return
+    if w in ("host", "tall-skinny"):
+        print(json.dumps(leg_line(streamed(w, args.steps, args.warmup, not args.no_check), args.steps,
+                                  args.warmup)), flush=True)
+        return
+    if w == "reorder":
+        print(json.dumps(leg_line(reorder(args.steps, args.warmup), args.steps, args.warmup)), flush=True)
+        return
+    if w == "batched":
+        print(json.dumps(leg_line(batched(args.steps, args.warmup), args.steps, args.warmup)), flush=True)
+        return
+    if w == "miniapp":
+        print(json.dumps(leg_line(miniapp(), args.steps, args.warmup)), flush=True)
+        return
+
+    out = headline(args)
     if not args.no_split:
         out["split_precision"] = split_precision_target()
+    if not args.quick and w is None:
+        t0 = time.perf_counter()
+        out["configs"] = {
+            "1_miniapp_2000": miniapp(),
+            "2_host_resident_16384": streamed("host", 3, 1, not args.no_check),
+            "2_reorder_16384": reorder(5, 2),
+            "3_tall_skinny_2048x2048x1048576": streamed("tall-skinny", 2, 1, not args.no_check),
+            "4_split_precision_8192": split_precision(steps=3, warmup=1, sweep=False),
+            "5_batched_4096": batched(10, 2),
+        }
+        out["configs_wall_s"] = round(time.perf_counter() - t0, 1)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n)
     print(json.dumps(out), flush=True)

@@ -82,6 +82,24 @@ def build(verbose=False):
     return LIB
 
 
+MINIAPP_SRC = os.path.join(HERE, "miniapp", "cosma_miniapp.cpp")
+MINIAPP = os.path.join(HERE, "bin", "cosma_miniapp")
+
+
+def build_miniapp(verbose=False):
+    """Links cosma_amd/bin/cosma_miniapp (the cosma::multiply driver) against libcosma_amd.so."""
+    os.makedirs(os.path.dirname(MINIAPP), exist_ok=True)
+    tmp = MINIAPP + ".tmp"
+    cmd = [CXX, *HOST, "-o", tmp, MINIAPP_SRC, "-L", HERE, "-l:libcosma_amd.so", "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, MINIAPP)
+    if verbose:
+        print(f"built {MINIAPP}")
+    return MINIAPP
+
+
 TEST_DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "front_test.cpp")
 TEST_DRIVER = os.path.join(ROOT, "tests", "cpp", "front_test")
 
@@ -139,6 +157,7 @@ def build_interpose_tests(verbose=False):
 
 if __name__ == "__main__":
     build(verbose=True)
+    build_miniapp(verbose=True)
     build_test_driver(verbose=True)
     build_interpose_tests(verbose=True)
     sys.exit(0)

@@ -143,7 +143,7 @@ def test_check_after_operands_change(handle):
     import torch
     import cosma_amd as ca
     probs = make_problems(12, 31, max_dim=120)
-    probs = [p for p in probs if p["k"] > 0 and p["alpha"] != 0.0 and p["A"].any()]
+    probs = [p for p in probs if p["k"] >= 4 and p["alpha"] != 0.0 and p["A"].any()]
     reset_c(probs)
     plan = ca.GroupedOzaki(handle, descs(probs))
     base = plan.info()["moduli"]
