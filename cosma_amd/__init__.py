@@ -87,6 +87,9 @@ SIGNATURES = {
     "cosma_amd_dgemm_ozaki": (ctypes.c_int, [c_vp, c_char, c_char, c_i64, c_i64, c_i64, c_dbl,
                                              c_vp, c_i64, c_vp, c_i64, c_dbl, c_vp, c_i64, ctypes.c_int,
                                              c_dbl, ctypes.c_int, ctypes.c_void_p, c_vp]),
+    "cosma_amd_dgemm_ozaki_noalloc": (ctypes.c_int, [c_vp, c_char, c_char, c_i64, c_i64, c_i64, c_dbl,
+                                                     c_vp, c_i64, c_vp, c_i64, c_dbl, c_vp, c_i64, ctypes.c_int,
+                                                     c_dbl, ctypes.c_int, ctypes.c_void_p, c_vp]),
     "cosma_amd_dgemm_host": (ctypes.c_int, [c_vp, c_char, c_char, c_i64, c_i64, c_i64, c_dbl,
                                             c_vp, c_i64, c_vp, c_i64, c_dbl, c_vp, c_i64, ctypes.c_int,
                                             ctypes.c_int]),
@@ -202,6 +205,15 @@ class Handle:
         check(lib().cosma_amd_dgemm_ozaki(self.ptr, _ch(transa), _ch(transb), m, n, k, alpha, a, lda, b, ldb,
                                           beta, c, ldc, method, tolerance, count, ctypes.addressof(info),
                                           stream))
+        return info.as_dict()
+
+    def dgemm_ozaki_noalloc(self, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,
+                            method=OZAKI_MODULI, tolerance=1e-14, count=0, stream=None):
+        """The split-precision GEMM without allocation (library scratch); returns the info dict."""
+        info = OzakiInfo()
+        check(lib().cosma_amd_dgemm_ozaki_noalloc(self.ptr, _ch(transa), _ch(transb), m, n, k, alpha, a, lda, b,
+                                                  ldb, beta, c, ldc, method, tolerance, count,
+                                                  ctypes.addressof(info), stream))
         return info.as_dict()
 
     def dgemm_host(self, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc,

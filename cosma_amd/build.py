@@ -40,11 +40,21 @@ COMMON = [
 HOST = ["-std=c++17", "-O2", "-fPIC", "-Wall", "-I", os.path.join(ROOT, "include"), "-I", FRONT]
 
 
+HOSTSRC = os.path.join(CSRC, "host")
+
+
+def _host_sources():
+    """Plain C++ sources of the library (compiled by g++): the COSMA
+    interface (csrc/front) and the host-side helpers (csrc/host)."""
+    out = [os.path.join(FRONT, n) for n in sorted(os.listdir(FRONT))
+           if n.endswith(".cpp") and os.path.join(FRONT, n) != INTERPOSE]
+    out += [os.path.join(HOSTSRC, n) for n in sorted(os.listdir(HOSTSRC)) if n.endswith(".cpp")]
+    return out
+
+
 def _sources():
     out = [os.path.join(CSRC, n) for n in sorted(os.listdir(CSRC)) if n.endswith(".hip")]
-    out += [os.path.join(FRONT, n) for n in sorted(os.listdir(FRONT))
-            if n.endswith(".cpp") and os.path.join(FRONT, n) != INTERPOSE]
-    return out
+    return out + _host_sources()
 
 
 def _compile(src):
@@ -65,7 +75,7 @@ def build(verbose=False):
     with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-ldl"]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-ldl", "-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
@@ -119,6 +129,47 @@ def build_test_driver(verbose=False):
 
 
 CPP_TESTS = os.path.join(ROOT, "tests", "cpp")
+SANITIZERS = {"asan": "address,undefined", "tsan": "thread"}
+
+
+def sanitized_exe(name, kind):
+    return os.path.join(CPP_TESTS, f"{name}_{kind}")
+
+
+def build_sanitized(verbose=False):
+    """Host-only sanitizer builds (g++, no device code) of the C++ interface,
+    the host-side helpers and the CPU oracle, driven by
+    tests/cpp/front_test.cpp and tests/cpp/sanitize_test.cpp:
+      front_test_asan      the COSMA interface (front/*.cpp, host/*.cpp)
+                           under AddressSanitizer + UBSan, linked against
+                           libcosma_amd.so for the C ABI it calls;
+      sanitize_test_asan   the host packing (host/*.cpp) and the oracle
+                           (oracle/oracle.c) under AddressSanitizer + UBSan;
+      sanitize_test_tsan   the threaded host packing under ThreadSanitizer.
+    The reference's own CI runs its tests in an AddressSanitizer image
+    (docker/asan/deploy.Dockerfile); GPU code is never built with sanitizers."""
+    out = {}
+    common = ["-g", "-O1", "-fno-omit-frame-pointer", "-fno-sanitize-recover=all"]
+    inc = ["-I", os.path.join(ROOT, "include"), "-I", FRONT, "-I", CSRC]
+    for kind in ("asan", "tsan"):
+        san = ["-fsanitize=" + SANITIZERS[kind]]
+        exe = sanitized_exe("sanitize_test", kind)
+        cmd = [CXX, "-std=c++17", *common, *san, *inc, "-o", exe + ".tmp", os.path.join(CPP_TESTS, "sanitize_test.cpp"),
+               *[os.path.join(HOSTSRC, n) for n in sorted(os.listdir(HOSTSRC)) if n.endswith(".cpp")],
+               "-x", "c", os.path.join(ROOT, "oracle", "oracle.c"), "-x", "none", "-lpthread", "-lm"]
+        _run(cmd)
+        os.replace(exe + ".tmp", exe)
+        out["sanitize_test_" + kind] = exe
+    exe = sanitized_exe("front_test", "asan")
+    cmd = [CXX, "-std=c++17", *common, "-fsanitize=" + SANITIZERS["asan"], *inc, "-o", exe + ".tmp",
+           TEST_DRIVER_SRC, *_host_sources(), "-L", HERE, "-l:libcosma_amd.so", "-ldl", "-lpthread",
+           "-Wl,-rpath,$ORIGIN/../../cosma_amd"]
+    _run(cmd)
+    os.replace(exe + ".tmp", exe)
+    out["front_test_asan"] = exe
+    if verbose:
+        print(f"built the sanitizer drivers in {CPP_TESTS}")
+    return out
 STUB_LIB = os.path.join(CPP_TESTS, "libstub_scalapack.so")
 # interposition test executables: name -> libraries in link (= lookup) order
 INTERPOSE_LINKS = {

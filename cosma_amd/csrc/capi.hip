@@ -59,7 +59,16 @@ int cosma_amd_handle_create(cosma_amd_handle_t *handle, int device, int streams,
         return COSMA_AMD_ERR_NO_DEVICE;
     }
     CA_HIP(hipSetDevice(device));
+    // the no-allocation scratch of this device, resolved with the device
+    // current; if that fails, only the no-allocation entry point refuses to run
+    void *scratch_addr = nullptr;
+    if (ozaki_scratch_address(device, &scratch_addr) != COSMA_AMD_OK) {
+        scratch_addr = nullptr;
+        (void)hipGetLastError();
+        clear_error();
+    }
     auto *h = new cosma_amd_handle_s();
+    h->ozaki_scratch = scratch_addr;
     h->device = device;
     h->n_streams = streams > 0 ? streams : 2;
     h->tile_m = tile_m > 0 ? tile_m : 5000;
@@ -160,11 +169,33 @@ int cosma_amd_dgemm_host(cosma_amd_handle_t h, char transa, char transb,
     return stream_dgemm(h, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, copy_c_back != 0);
 }
 
+static int dgemm_ozaki_entry(cosma_amd_handle_t h, char transa, char transb, int64_t m, int64_t n, int64_t k,
+                             double alpha, const double *a, int64_t lda, const double *b, int64_t ldb,
+                             double beta, double *c, int64_t ldc, int method, double tolerance, int count,
+                             cosma_amd_ozaki_info *info, void *stream, bool noalloc);
+
 int cosma_amd_dgemm_ozaki(cosma_amd_handle_t h, char transa, char transb,
                           int64_t m, int64_t n, int64_t k, double alpha,
                           const double *a, int64_t lda, const double *b, int64_t ldb,
                           double beta, double *c, int64_t ldc, int method, double tolerance,
                           int count, cosma_amd_ozaki_info *info, void *stream) {
+    return dgemm_ozaki_entry(h, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, method, tolerance,
+                             count, info, stream, false);
+}
+
+int cosma_amd_dgemm_ozaki_noalloc(cosma_amd_handle_t h, char transa, char transb,
+                                  int64_t m, int64_t n, int64_t k, double alpha,
+                                  const double *a, int64_t lda, const double *b, int64_t ldb,
+                                  double beta, double *c, int64_t ldc, int method, double tolerance,
+                                  int count, cosma_amd_ozaki_info *info, void *stream) {
+    return dgemm_ozaki_entry(h, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, method, tolerance,
+                             count, info, stream, true);
+}
+
+static int dgemm_ozaki_entry(cosma_amd_handle_t h, char transa, char transb, int64_t m, int64_t n, int64_t k,
+                             double alpha, const double *a, int64_t lda, const double *b, int64_t ldb,
+                             double beta, double *c, int64_t ldc, int method, double tolerance, int count,
+                             cosma_amd_ozaki_info *info, void *stream, bool noalloc) {
     clear_error();
     CA_CHECK(h != nullptr, "dgemm_ozaki: null handle");
     const char ta = norm_trans(transa), tb = norm_trans(transb);
@@ -193,6 +224,9 @@ int cosma_amd_dgemm_ozaki(cosma_amd_handle_t h, char transa, char transb,
         }
     }
     CA_HIP(hipSetDevice(h->device));
+    if (noalloc)
+        return ozaki_dgemm_noalloc(h, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, method, tolerance,
+                                   count, info, pick_stream(h, stream));
     return ozaki_dgemm(h, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, method, tolerance, count,
                        info, pick_stream(h, stream));
 }

@@ -85,6 +85,7 @@ struct cosma_amd_handle_s {
     cosma_amd::DevBuf ozaki_ws;         // split-precision workspace
     cosma_amd::DevBuf ozaki_aux;        // split-precision row statistics and exponents
     hipEvent_t ozaki_ev[2] = {nullptr, nullptr};   // around the split-precision product launch
+    void *ozaki_scratch = nullptr;      // device address of the no-allocation scratch array on this device
     std::vector<hipEvent_t> events;     // event pool
     size_t events_used = 0;
     std::vector<cosma_amd::StagingSlot> h2d_slots, d2h_slots;
@@ -158,9 +159,28 @@ void release_stream_state(cosma_amd_handle_t h);
 int layout_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha,
                  const cosma_amd_block *A, int64_t na, const cosma_amd_block *B, int64_t nb, double beta,
                  const cosma_amd_block *C, int64_t nc);
+// Caller-provided memory for a split-precision call that allocates nothing:
+// device scratch for the statistics and the residues / slices, and host
+// staging for the statistics read-back.
+struct OzakiFixedMemory {
+    void *device;
+    size_t device_bytes;
+    double *host;
+    size_t host_bytes;
+};
 int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha,
                 const double *a, int64_t lda, const double *b, int64_t ldb, double beta, double *c,
-                int64_t ldc, int method, double tol, int count, cosma_amd_ozaki_info *info, hipStream_t s);
+                int64_t ldc, int method, double tol, int count, cosma_amd_ozaki_info *info, hipStream_t s,
+                const OzakiFixedMemory *fixed = nullptr);
+// Device address of the split-precision scratch array on `device` (resolved
+// once per device, with that device current); the handle keeps it.
+int ozaki_scratch_address(int device, void **ptr);
+// The no-allocation split-precision call: ozaki_dgemm on the device's
+// scratch array and a process-wide host staging buffer (serialised).
+int ozaki_dgemm_noalloc(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha,
+                        const double *a, int64_t lda, const double *b, int64_t ldb, double beta, double *c,
+                        int64_t ldc, int method, double tol, int count, cosma_amd_ozaki_info *info,
+                        hipStream_t s);
 int moduli_bits(int n, int64_t k);
 int launch_fill_lognormal(double *x, int64_t rows, int64_t cols, int64_t ld, uint64_t seed, double phi,
                           hipStream_t s);

@@ -33,6 +33,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "ozaki.hpp"
@@ -506,9 +507,19 @@ struct ErrModel {
     double gamma = 0;       // a-priori relative estimate = gamma * 2^-P (doubled where used)
 };
 
-ErrModel error_model(const std::vector<int> &E, const std::vector<int> &F, const std::vector<double> &ksqA,
-                     const std::vector<double> &ksqB, const std::vector<double> &l1A,
-                     const std::vector<double> &l1B, int64_t k) {
+// A read-only view of n host values.
+template <typename T>
+struct View {
+    const T *p;
+    int64_t n;
+    const T &operator[](int64_t i) const { return p[i]; }
+    int64_t size() const { return n; }
+    const T *begin() const { return p; }
+    const T *end() const { return p + n; }
+};
+
+ErrModel error_model(View<int> E, View<int> F, View<double> ksqA, View<double> ksqB, View<double> l1A,
+                     View<double> l1B, int64_t k) {
     ErrModel em;
     em.k = k;
     int ea = INT_MIN, fb = INT_MIN;
@@ -521,20 +532,20 @@ ErrModel error_model(const std::vector<int> &E, const std::vector<int> &F, const
     em.ea = ea;
     em.fb = fb;
     double SA = 0.0, SB = 0.0, NA = 0.0, NB = 0.0, D = 0.0, LA = 0.0, LB = 0.0;
-    for (size_t i = 0; i < E.size(); ++i) {
+    for (int64_t i = 0; i < E.size(); ++i) {
         if (E[i] == ZERO_EXPONENT) continue;
         SA += std::ldexp(1.0, 2 * (E[i] - ea));
         const double a = std::ldexp(l1A[i], -ea);
         LA += a * a;
     }
-    for (size_t j = 0; j < F.size(); ++j) {
+    for (int64_t j = 0; j < F.size(); ++j) {
         if (F[j] == ZERO_EXPONENT) continue;
         SB += std::ldexp(1.0, 2 * (F[j] - fb));
         const double b = std::ldexp(l1B[j], -fb);
         LB += b * b;
     }
     bool finite = true;
-    for (size_t l = 0; l < ksqA.size(); ++l) {
+    for (int64_t l = 0; l < ksqA.size(); ++l) {
         if (!std::isfinite(ksqA[l]) || !std::isfinite(ksqB[l])) finite = false;
         const double a = std::ldexp(ksqA[l], -2 * ea), b = std::ldexp(ksqB[l], -2 * fb);
         NA += a;
@@ -574,7 +585,8 @@ double rigorous_num(const ErrModel &em, bool slices, int P_or_s) {
 
 int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha,
                 const double *a, int64_t lda, const double *b, int64_t ldb, double beta, double *c,
-                int64_t ldc, int method, double tol, int count, cosma_amd_ozaki_info *info, hipStream_t s) {
+                int64_t ldc, int method, double tol, int count, cosma_amd_ozaki_info *info, hipStream_t s,
+                const OzakiFixedMemory *fixed) {
     if (info) *info = cosma_amd_ozaki_info{};
     if (m == 0 || n == 0) return COSMA_AMD_OK;
     if (k == 0 || alpha == 0.0) return launch_scale(m, n, beta, c, ldc, s);
@@ -600,6 +612,29 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
     const int64_t Mp = round_up(m, 256), Np = round_up(n, 256), Kp = round_up(k, 256);
     const int64_t ldr = round_up(m, 16);                        // residue output ld
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    // Workspace layout (see steps 1 and 2 below): the statistics block `aux`,
+    // then per pass the slices / residues of A and B and the product output.
+    const int64_t nclear = 3 * (Mp + Np) + 2 * Kp + 1;
+    const size_t norm_off = (size_t)nclear * 8 + (size_t)(Mp + Np) * 4;
+    const size_t aux_bytes = al(norm_off + NORM_SLOTS * 8);
+    const int64_t a_plane = Mp * Kp * esize + PLANE_PAD, b_plane = Np * Kp * esize + PLANE_PAD;
+    const int64_t r_plane = ldr * n + PLANE_PAD;
+    auto ws_bytes = [&](int cnt) {
+        const size_t out_bytes = slices ? (size_t)ldr * n * 8 : (size_t)cnt * r_plane;
+        return al((size_t)cnt * a_plane) + al((size_t)cnt * b_plane) + al(out_bytes);
+    };
+    const size_t span = (size_t)(2 * Kp + 2 * (Mp + Np) + 1) * 8 + (size_t)(Mp + Np) * 4;   // statistics read back
+    if (fixed) {
+        // nothing is allocated: the largest count the call may use must fit
+        const int most = count > 0 ? count : (slices ? I8_MAX_PAIRS : MAX_MODULI);
+        if (aux_bytes + ws_bytes(most) > fixed->device_bytes || span > fixed->host_bytes) {
+            set_error("ozaki (no allocation): m = %lld, n = %lld, k = %lld with up to %d %s need %zu bytes of "
+                      "device scratch (%zu available) and %zu bytes of host staging (%zu available)",
+                      (long long)m, (long long)n, (long long)k, most, slices ? "slices" : "moduli",
+                      aux_bytes + ws_bytes(most), fixed->device_bytes, span, fixed->host_bytes);
+            return COSMA_AMD_ERR_UNSUPPORTED;
+        }
+    }
     auto grow = [](DevBuf &buf, size_t need) -> int {
         if (buf.bytes >= need) return COSMA_AMD_OK;
         if (buf.ptr) CA_HIP(hipFree(buf.ptr));
@@ -621,10 +656,14 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
     // rsqA [Mp], rsqB [Np], rabsA [Mp], rabsB [Np], flag [1] (cleared
     // together, 8 bytes each), then the exponents E [Mp] and F [Np], then
     // the NORM_SLOTS partial sums of the product's norm
-    const int64_t nclear = 3 * (Mp + Np) + 2 * Kp + 1;
-    const size_t norm_off = (size_t)nclear * 8 + (size_t)(Mp + Np) * 4;
-    if ((rc = grow(h->ozaki_aux, al(norm_off + NORM_SLOTS * 8)))) return rc;
-    unsigned long long *maxA = static_cast<unsigned long long *>(h->ozaki_aux.ptr);
+    char *aux = nullptr;
+    if (fixed) {
+        aux = static_cast<char *>(fixed->device);
+    } else {
+        if ((rc = grow(h->ozaki_aux, aux_bytes))) return rc;
+        aux = static_cast<char *>(h->ozaki_aux.ptr);
+    }
+    unsigned long long *maxA = reinterpret_cast<unsigned long long *>(aux);
     unsigned long long *maxB = maxA + Mp;
     double *ksqA = reinterpret_cast<double *>(maxB + Np);
     double *ksqB = ksqA + Kp;
@@ -635,7 +674,7 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
     int *flag = reinterpret_cast<int *>(rabsB + Np);
     int *E = reinterpret_cast<int *>(rabsB + Np + 1);
     int *F = E + Mp;
-    double *nrm = reinterpret_cast<double *>(static_cast<char *>(h->ozaki_aux.ptr) + norm_off);
+    double *nrm = reinterpret_cast<double *>(aux + norm_off);
     hipLaunchKernelGGL(clear_u64, dim3(grid_for(nclear)), dim3(256), 0, s, maxA, nclear);
     const bool a_rows_contig = ta == 'T', b_rows_contig = tb == 'N';   // op rows contiguous in k
     // grid: x over the contiguous dimension in 64s, y over the other in STATS_SPAN
@@ -653,19 +692,24 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
     // One read-back of the contiguous span ksqA .. F (each small copy costs
     // tens of microseconds of fixed overhead, which several copies paid in
     // turn while the GPU waited).
-    const size_t span = (size_t)(2 * Kp + 2 * (Mp + Np) + 1) * 8 + (size_t)(Mp + Np) * 4;
-    std::vector<double> hbuf((span + 7) / 8);
-    CA_HIP(hipMemcpyAsync(hbuf.data(), ksqA, span, hipMemcpyDeviceToHost, s));
+    std::vector<double> hbuf;
+    double *hb = nullptr;
+    if (fixed) {
+        hb = fixed->host;
+    } else {
+        hbuf.resize((span + 7) / 8);
+        hb = hbuf.data();
+    }
+    CA_HIP(hipMemcpyAsync(hb, ksqA, span, hipMemcpyDeviceToHost, s));
     CA_HIP(hipStreamSynchronize(s));
-    const double *hb = hbuf.data();
-    const std::vector<double> hkA(hb, hb + k), hkB(hb + Kp, hb + Kp + k);
-    const std::vector<double> hrA(hb + 2 * Kp, hb + 2 * Kp + m), hrB(hb + 2 * Kp + Mp, hb + 2 * Kp + Mp + n);
+    const View<double> hkA{hb, k}, hkB{hb + Kp, k};
+    const View<double> hrA{hb + 2 * Kp, m}, hrB{hb + 2 * Kp + Mp, n};
     const double *hl = hb + 2 * Kp + Mp + Np;
-    const std::vector<double> hlA(hl, hl + m), hlB(hl + Mp, hl + Mp + n);
+    const View<double> hlA{hl, m}, hlB{hl + Mp, n};
     int nonfinite;
     std::memcpy(&nonfinite, hl + Mp + Np, sizeof(int));
     const int *hEp = reinterpret_cast<const int *>(hl + Mp + Np + 1);
-    const std::vector<int> hE(hEp, hEp + m), hF(hEp + Mp, hEp + Mp + n);
+    const View<int> hE{hEp, m}, hF{hEp + Mp, n};
     if (nonfinite) {
         // Inf / NaN cannot be split; the FP64 MFMA DGEMM propagates them as IEEE does.
         if (info) {
@@ -693,9 +737,9 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
     // squares of a vector's largest entries neither overflow nor underflow
     // (entries that do underflow are negligible beside them); otherwise P
     // comes from the entrywise bound alone.
-    auto norm_max = [&](const std::vector<int> &ex, const std::vector<double> &rs) {
+    auto norm_max = [&](View<int> ex, View<double> rs) {
         double mx = 0.0;
-        for (size_t i = 0; i < ex.size(); ++i) {
+        for (int64_t i = 0; i < ex.size(); ++i) {
             if (ex[i] == ZERO_EXPONENT) continue;
             const double v = (ex[i] < -NORM_EXP_LIMIT || ex[i] > NORM_EXP_LIMIT) ? INFINITY
                                                                                   : std::ldexp(rs[i], -2 * ex[i]);
@@ -744,9 +788,12 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
     }
 
     // the INT8 product kernel's duration (for the caller's roofline), timed
-    // by two events around its launch
-    for (auto &ev : h->ozaki_ev)
-        if (!ev) CA_HIP(hipEventCreate(&ev));
+    // by two events around its launch (not with fixed memory, which creates
+    // nothing)
+    const bool timing = !fixed;
+    if (timing)
+        for (auto &ev : h->ozaki_ev)
+            if (!ev) CA_HIP(hipEventCreate(&ev));
     const int shift = em.zero ? 0 : em.ea + em.fb + (int)std::ceil(std::log2((double)k + 1.0));
     int64_t products = 0;
     int passes = 0;
@@ -772,12 +819,14 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
         // at addresses 2^26 apart, and the split stores and recombination loads,
         // which touch the same offset of every plane together, would pile onto
         // the same memory channels.
-        const int64_t a_plane = Mp * Kp * esize + PLANE_PAD, b_plane = Np * Kp * esize + PLANE_PAD;
-        const int64_t r_plane = ldr * n + PLANE_PAD;
         const size_t a_bytes = (size_t)count * a_plane, b_bytes = (size_t)count * b_plane;
-        const size_t out_bytes = slices ? (size_t)ldr * n * 8 : (size_t)count * r_plane;
-        if ((rc = grow(h->ozaki_ws, al(a_bytes) + al(b_bytes) + al(out_bytes)))) return rc;
-        char *ws = static_cast<char *>(h->ozaki_ws.ptr);
+        char *ws = nullptr;
+        if (fixed) {
+            ws = static_cast<char *>(fixed->device) + aux_bytes;
+        } else {
+            if ((rc = grow(h->ozaki_ws, ws_bytes(count)))) return rc;
+            ws = static_cast<char *>(h->ozaki_ws.ptr);
+        }
         int8_t *As = reinterpret_cast<int8_t *>(ws);
         int8_t *Bs = reinterpret_cast<int8_t *>(ws + al(a_bytes));
         void *out = ws + al(a_bytes) + al(b_bytes);
@@ -842,15 +891,15 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
                 jobs.pb[l][0] = (unsigned char)l;
                 jobs.modulus[l] = kModuli[l];
             }
-            CA_HIP(hipEventRecord(h->ozaki_ev[0], s));
+            if (timing) CA_HIP(hipEventRecord(h->ozaki_ev[0], s));
             if ((rc = launch_i8gemm(pr, jobs, count, EPI_RES8, s))) return rc;
-            CA_HIP(hipEventRecord(h->ozaki_ev[1], s));
+            if (timing) CA_HIP(hipEventRecord(h->ozaki_ev[1], s));
             pass_products = count;
             crt_constants(count, cc);
             recombine(mode);
         } else {
             pr.out_stride = 0;
-            CA_HIP(hipEventRecord(h->ozaki_ev[0], s));
+            if (timing) CA_HIP(hipEventRecord(h->ozaki_ev[0], s));
             // weight d = t + u (slice indices from 1): pairs with t + u = d, d <= count + 1
             for (int d = 2; d <= count + 1; ++d) {
                 I8Jobs jd{};
@@ -868,7 +917,7 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
                 if ((rc = launch_i8gemm(pr, jd, 1, EPI_F64ACC, s))) return rc;
                 pass_products += np;
             }
-            CA_HIP(hipEventRecord(h->ozaki_ev[1], s));
+            if (timing) CA_HIP(hipEventRecord(h->ozaki_ev[1], s));
             slices_out(mode);
         }
         CA_HIP(hipGetLastError());
@@ -876,9 +925,11 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
         double parts[NORM_SLOTS];
         CA_HIP(hipMemcpyAsync(parts, nrm, sizeof(parts), hipMemcpyDeviceToHost, s));
         CA_HIP(hipStreamSynchronize(s));
-        float ms = 0.0f;
-        CA_HIP(hipEventElapsedTime(&ms, h->ozaki_ev[0], h->ozaki_ev[1]));
-        product_ms += ms;
+        if (timing) {
+            float ms = 0.0f;
+            CA_HIP(hipEventElapsedTime(&ms, h->ozaki_ev[0], h->ozaki_ev[1]));
+            product_ms += ms;
+        }
         double sum = 0.0;
         for (double v : parts) sum += v;
         nx = std::sqrt(sum) * std::ldexp(1.0, shift - (em.ea + em.fb));   // ||X'||_F / 2^(ea + fb)
@@ -910,6 +961,60 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
         info->product_ms = product_ms;
     }
     return COSMA_AMD_OK;
+}
+
+// ------------------------------------------------ no-allocation entry point
+// Device scratch of the no-allocation split-precision call: the statistics
+// and the residues / slices of one call, 64 MiB on every device the library
+// runs on (held until the process ends).  Host code never uses the array's
+// name as an address: each device's address comes from hipGetSymbolAddress
+// (ozaki_scratch_address) and is what the kernels receive.
+constexpr size_t OZAKI_SCRATCH_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[OZAKI_SCRATCH_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_addr[MAX_DEVICES] = {};
+// Host staging of the statistics read-back (shared by all no-allocation
+// calls of the process, which it serialises).
+constexpr size_t OZAKI_HOST_STAGING_BYTES = (size_t)1 << 20;
+double g_host_staging[OZAKI_HOST_STAGING_BYTES / 8];
+std::mutex g_host_staging_mu;
+}  // namespace
+
+int ozaki_scratch_address(int device, void **ptr) {
+    *ptr = nullptr;
+    if (device < 0 || device >= MAX_DEVICES) {
+        set_error("ozaki scratch: device %d beyond the %d supported", device, MAX_DEVICES);
+        return COSMA_AMD_ERR_UNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[device]) {
+        int cur = 0;
+        CA_HIP(hipGetDevice(&cur));
+        CA_HIP(hipSetDevice(device));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p) return hip_fail(e, "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+        g_scratch_addr[device] = p;
+    }
+    *ptr = g_scratch_addr[device];
+    return COSMA_AMD_OK;
+}
+
+int ozaki_dgemm_noalloc(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha,
+                        const double *a, int64_t lda, const double *b, int64_t ldb, double beta, double *c,
+                        int64_t ldc, int method, double tol, int count, cosma_amd_ozaki_info *info,
+                        hipStream_t s) {
+    if (!h->ozaki_scratch) {
+        set_error("dgemm_ozaki_noalloc: the handle has no scratch address for device %d", h->device);
+        return COSMA_AMD_ERR_UNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> lock(g_host_staging_mu);
+    const OzakiFixedMemory mem{h->ozaki_scratch, OZAKI_SCRATCH_BYTES, g_host_staging, OZAKI_HOST_STAGING_BYTES};
+    return ozaki_dgemm(h, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, method, tol, count, info, s, &mem);
 }
 
 }  // namespace cosma_amd

@@ -45,6 +45,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "host/host_copy.hpp"
 
 namespace cosma_amd {
 namespace {
@@ -133,26 +134,6 @@ int ensure_slots(std::vector<StagingSlot> &slots) {
     return COSMA_AMD_OK;
 }
 
-// Column-major 2D copy between host buffers, split over threads by columns.
-void host_copy_2d(double *dst, int64_t dld, const double *src, int64_t sld, int64_t rows, int64_t cols) {
-    const size_t bytes = (size_t)rows * cols * 8;
-    int nt = (int)std::min<size_t>(8, std::max<size_t>(1, bytes >> 22));   // one thread per 4 MB
-    nt = (int)std::min<int64_t>(nt, cols);
-    auto work = [&](int64_t c0, int64_t c1) {
-        for (int64_t c = c0; c < c1; ++c) std::memcpy(dst + c * dld, src + c * sld, rows * 8);
-    };
-    if (nt <= 1) {
-        work(0, cols);
-        return;
-    }
-    std::vector<std::thread> th;
-    const int64_t per = (cols + nt - 1) / nt;
-    for (int t = 0; t < nt; ++t) {
-        const int64_t c0 = t * per, c1 = std::min(cols, c0 + per);
-        if (c0 < c1) th.emplace_back(work, c0, c1);
-    }
-    for (auto &x : th) x.join();
-}
 
 // Finishes a pending copy-out of a D2H staging slot into pageable memory.
 int drain_slot(StagingSlot &s) {

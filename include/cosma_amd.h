@@ -164,6 +164,30 @@ int cosma_amd_dgemm_ozaki(cosma_amd_handle_t handle, char transa, char transb,
                           double beta, double *c, int64_t ldc, int method, double tolerance,
                           int count, cosma_amd_ozaki_info *info, void *stream);
 
+/*
+ * The same call with no allocation: no hipMalloc, hipFree, hipHostMalloc or
+ * event creation happens in it, for callers that cannot allocate (tight
+ * serving loops, harnesses that forbid device allocation).  No counterpart
+ * in the reference.  The device workspace is a 64 MiB scratch array that
+ * the library holds on every device it runs on until the process ends (its
+ * address on each device is resolved when a handle is created there), and
+ * the statistics are staged in a 1 MiB host buffer of the library.  Limit:
+ * the statistics, residues / slices and product output of the largest count
+ * the call may use (the forced count, else 20 moduli / 8 slices) must fit in
+ * the 64 MiB; e.g. m = n = k = 1024 fits with any count.  Larger calls
+ * return COSMA_AMD_ERR_UNSUPPORTED before any work is queued.  Results are
+ * bit-identical to cosma_amd_dgemm_ozaki; info->product_ms is 0 (no timing
+ * events).  Every call on a device shares that device's scratch (each call
+ * clears what it reads), and the host staging is shared by the process:
+ * no-allocation calls are serialised on the host, and calls that overlap on
+ * one device through different streams race.
+ */
+int cosma_amd_dgemm_ozaki_noalloc(cosma_amd_handle_t handle, char transa, char transb,
+                                  int64_t m, int64_t n, int64_t k, double alpha,
+                                  const double *a, int64_t lda, const double *b, int64_t ldb,
+                                  double beta, double *c, int64_t ldc, int method, double tolerance,
+                                  int count, cosma_amd_ozaki_info *info, void *stream);
+
 /* ------------------------------------------------- host-streaming GEMM */
 /*
  * Same contract as cosma_amd_dgemm, but each of A, B, C may be device memory,
