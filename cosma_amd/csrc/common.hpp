@@ -84,6 +84,7 @@ struct cosma_amd_handle_s {
     cosma_amd::DevBuf dC0;              // ring of incoming C tiles (deferred beta * C)
     cosma_amd::DevBuf ozaki_ws;         // split-precision workspace
     cosma_amd::DevBuf ozaki_aux;        // split-precision row statistics and exponents
+    cosma_amd::DevBuf ozaki_host;       // page-locked host copy of the statistics (hipHostMalloc)
     hipEvent_t ozaki_ev[2] = {nullptr, nullptr};   // around the split-precision product launch
     void *ozaki_scratch = nullptr;      // device address of the no-allocation scratch array on this device
     std::vector<hipEvent_t> events;     // event pool

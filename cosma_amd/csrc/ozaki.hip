@@ -30,6 +30,7 @@
 // The number of moduli (or slices) is chosen from the requested tolerance; the
 // a-priori bound and the count used are returned to the caller.
 #include <algorithm>
+#include <cfloat>
 #include <climits>
 #include <cmath>
 #include <cstring>
@@ -507,6 +508,31 @@ struct ErrModel {
     double gamma = 0;       // a-priori relative estimate = gamma * 2^-P (doubled where used)
 };
 
+// Powers of two 2^e for e in [-1100, 1099] (0 or inf outside the double range).
+const double *pow2_table() {
+    static const std::vector<double> t = [] {
+        std::vector<double> v(2200);
+        for (int e = -1100; e < 1100; ++e) v[e + 1100] = std::ldexp(1.0, e);
+        return v;
+    }();
+    return t.data() + 1100;
+}
+
+// std::ldexp(x, e) without its per-call cost (tens of thousands of calls per
+// product made the host statistics take ~0.4 ms at 8192^3): one
+// multiplication by an exact power of two when that power and the product
+// are normal numbers, where the product is exact and equal to ldexp's;
+// ldexp itself otherwise (subnormal, zero, infinite or NaN results).
+// (P2 = pow2_table(), fetched once outside the loops.)
+inline double scale2(const double *P2, double x, int e) {
+    if (e > -1022 && e < 1024) {
+        const double r = x * P2[e];
+        const double a = std::fabs(r);
+        if (a >= DBL_MIN && a <= DBL_MAX) return r;
+    }
+    return std::ldexp(x, e);
+}
+
 // A read-only view of n host values.
 template <typename T>
 struct View {
@@ -531,23 +557,24 @@ ErrModel error_model(View<int> E, View<int> F, View<double> ksqA, View<double> k
     }
     em.ea = ea;
     em.fb = fb;
+    const double *P2 = pow2_table();
     double SA = 0.0, SB = 0.0, NA = 0.0, NB = 0.0, D = 0.0, LA = 0.0, LB = 0.0;
     for (int64_t i = 0; i < E.size(); ++i) {
         if (E[i] == ZERO_EXPONENT) continue;
-        SA += std::ldexp(1.0, 2 * (E[i] - ea));
-        const double a = std::ldexp(l1A[i], -ea);
+        SA += scale2(P2, 1.0, 2 * (E[i] - ea));
+        const double a = scale2(P2, l1A[i], -ea);
         LA += a * a;
     }
     for (int64_t j = 0; j < F.size(); ++j) {
         if (F[j] == ZERO_EXPONENT) continue;
-        SB += std::ldexp(1.0, 2 * (F[j] - fb));
-        const double b = std::ldexp(l1B[j], -fb);
+        SB += scale2(P2, 1.0, 2 * (F[j] - fb));
+        const double b = scale2(P2, l1B[j], -fb);
         LB += b * b;
     }
     bool finite = true;
     for (int64_t l = 0; l < ksqA.size(); ++l) {
         if (!std::isfinite(ksqA[l]) || !std::isfinite(ksqB[l])) finite = false;
-        const double a = std::ldexp(ksqA[l], -2 * ea), b = std::ldexp(ksqB[l], -2 * fb);
+        const double a = scale2(P2, ksqA[l], -2 * ea), b = scale2(P2, ksqB[l], -2 * fb);
         NA += a;
         NB += b;
         D += a * b;
@@ -624,14 +651,16 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
         return al((size_t)cnt * a_plane) + al((size_t)cnt * b_plane) + al(out_bytes);
     };
     const size_t span = (size_t)(2 * Kp + 2 * (Mp + Np) + 1) * 8 + (size_t)(Mp + Np) * 4;   // statistics read back
+    // host staging: the statistics, then the NORM_SLOTS partial norms
+    const size_t host_bytes = (span + 7) / 8 * 8 + NORM_SLOTS * 8;
     if (fixed) {
         // nothing is allocated: the largest count the call may use must fit
         const int most = count > 0 ? count : (slices ? I8_MAX_PAIRS : MAX_MODULI);
-        if (aux_bytes + ws_bytes(most) > fixed->device_bytes || span > fixed->host_bytes) {
+        if (aux_bytes + ws_bytes(most) > fixed->device_bytes || host_bytes > fixed->host_bytes) {
             set_error("ozaki (no allocation): m = %lld, n = %lld, k = %lld with up to %d %s need %zu bytes of "
                       "device scratch (%zu available) and %zu bytes of host staging (%zu available)",
                       (long long)m, (long long)n, (long long)k, most, slices ? "slices" : "moduli",
-                      aux_bytes + ws_bytes(most), fixed->device_bytes, span, fixed->host_bytes);
+                      aux_bytes + ws_bytes(most), fixed->device_bytes, host_bytes, fixed->host_bytes);
             return COSMA_AMD_ERR_UNSUPPORTED;
         }
     }
@@ -691,14 +720,26 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
     CA_HIP(hipGetLastError());
     // One read-back of the contiguous span ksqA .. F (each small copy costs
     // tens of microseconds of fixed overhead, which several copies paid in
-    // turn while the GPU waited).
-    std::vector<double> hbuf;
+    // turn while the GPU waited), into page-locked memory kept by the handle:
+    // into pageable memory the runtime's staging added ~0.3 ms per call at
+    // 8192^3.
     double *hb = nullptr;
     if (fixed) {
         hb = fixed->host;
     } else {
-        hbuf.resize((span + 7) / 8);
-        hb = hbuf.data();
+        if (h->ozaki_host.bytes < host_bytes) {
+            if (h->ozaki_host.ptr) CA_HIP(hipHostFree(h->ozaki_host.ptr));
+            h->ozaki_host.ptr = nullptr;
+            h->ozaki_host.bytes = 0;
+            const hipError_t e = hipHostMalloc(&h->ozaki_host.ptr, host_bytes, hipHostMallocDefault);
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                set_error("ozaki: page-locked statistics buffer of %zu bytes failed", host_bytes);
+                return COSMA_AMD_ERR_ALLOC;
+            }
+            h->ozaki_host.bytes = host_bytes;
+        }
+        hb = static_cast<double *>(h->ozaki_host.ptr);
     }
     CA_HIP(hipMemcpyAsync(hb, ksqA, span, hipMemcpyDeviceToHost, s));
     CA_HIP(hipStreamSynchronize(s));
@@ -738,11 +779,12 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
     // (entries that do underflow are negligible beside them); otherwise P
     // comes from the entrywise bound alone.
     auto norm_max = [&](View<int> ex, View<double> rs) {
+        const double *P2 = pow2_table();
         double mx = 0.0;
         for (int64_t i = 0; i < ex.size(); ++i) {
             if (ex[i] == ZERO_EXPONENT) continue;
             const double v = (ex[i] < -NORM_EXP_LIMIT || ex[i] > NORM_EXP_LIMIT) ? INFINITY
-                                                                                  : std::ldexp(rs[i], -2 * ex[i]);
+                                                                                  : scale2(P2, rs[i], -2 * ex[i]);
             mx = v > mx ? v : (v != v ? INFINITY : mx);
         }
         return std::sqrt(mx) * (1.0 + 1e-9);
@@ -922,8 +964,8 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
         }
         CA_HIP(hipGetLastError());
         products += pass_products;
-        double parts[NORM_SLOTS];
-        CA_HIP(hipMemcpyAsync(parts, nrm, sizeof(parts), hipMemcpyDeviceToHost, s));
+        double *parts = hb + (span + 7) / 8;   // NORM_SLOTS partial norms, after the statistics
+        CA_HIP(hipMemcpyAsync(parts, nrm, NORM_SLOTS * sizeof(double), hipMemcpyDeviceToHost, s));
         CA_HIP(hipStreamSynchronize(s));
         if (timing) {
             float ms = 0.0f;
@@ -931,7 +973,7 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
             product_ms += ms;
         }
         double sum = 0.0;
-        for (double v : parts) sum += v;
+        for (int q = 0; q < NORM_SLOTS; ++q) sum += parts[q];
         nx = std::sqrt(sum) * std::ldexp(1.0, shift - (em.ea + em.fb));   // ||X'||_F / 2^(ea + fb)
         met = meets(count, nx);
         const bool last = met || forced || count >= max_count || passes >= 3;

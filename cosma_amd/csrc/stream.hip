@@ -263,6 +263,9 @@ void release_stream_state(cosma_amd_handle_t h) {
         b->ptr = nullptr;
         b->bytes = 0;
     }
+    if (h->ozaki_host.ptr) hipHostFree(h->ozaki_host.ptr);
+    h->ozaki_host.ptr = nullptr;
+    h->ozaki_host.bytes = 0;
     for (auto *v : {&h->h2d_slots, &h->d2h_slots}) {
         for (auto &s : *v) {
             if (s.ev) hipEventDestroy(s.ev);
