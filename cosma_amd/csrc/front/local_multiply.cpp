@@ -38,4 +38,11 @@ void local_multiply<double>(double *a, double *b, double *c, int m, int n, int k
     local_multiply<double>(get_context_instance<double>(), a, b, c, m, n, k, alpha, beta, copy_c_back);
 }
 
+// The reference's naive host loop (src/cosma/local_multiply.cpp:277-297):
+// same arguments and result, computed by the MI355X DGEMM with C copied back.
+template <>
+void local_multiply_cpu<double>(double *a, double *b, double *c, int m, int n, int k, double alpha, double beta) {
+    local_multiply<double>(get_context_instance<double>(), a, b, c, m, n, k, alpha, beta, true);
+}
+
 }  // namespace cosma

@@ -7,8 +7,12 @@
 // the FP64 MFMA DGEMM directly when all three buffers are on the device,
 // otherwise the host-streaming GEMM (resident panels or a bounded ring,
 // overlapped uploads, per-tile DGEMMs and downloads).  There is no CPU
-// branch; local_multiply_cpu, the reference's naive test loop, is not part of
-// the library.  Errors throw std::runtime_error.
+// branch: local_multiply_cpu, the reference's naive loop that its test
+// utilities call for the expected result (utils/cosma_utils.hpp:276-283),
+// keeps its signature and contract (host buffers, C <- alpha A B + beta C)
+// but runs on the MI355X like the other overloads, so its result equals the
+// naive loop's up to FP64 rounding, not bit for bit.  Errors throw
+// std::runtime_error.
 #pragma once
 #include <cosma/context.hpp>
 
@@ -26,6 +30,11 @@ void local_multiply(context<Scalar> &ctx, Scalar *a, Scalar *b, Scalar *c, int m
 template <typename Scalar>
 void local_multiply(Scalar *a, Scalar *b, Scalar *c, int m, int n, int k, Scalar alpha, Scalar beta,
                     bool copy_c_back);
+
+// src/cosma/local_multiply.hpp:18-26: host buffers, C overwritten in place
+// (uses get_context_instance<Scalar>()).
+template <typename Scalar>
+void local_multiply_cpu(Scalar *a, Scalar *b, Scalar *c, int m, int n, int k, Scalar alpha, Scalar beta);
 
 // On an explicit device handle (the reference's gpu::mm_handle overload);
 // with copy_c_back == false the result stays on the device (see

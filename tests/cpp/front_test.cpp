@@ -5,7 +5,7 @@
 //       one line per check.
 //   front_test local_multiply <variant> m n k alpha beta out.bin
 //       cosma::local_multiply on host buffers; variant = singleton | context |
-//       handle.  A, B, C are the counter-based uniform matrices with seeds
+//       handle | cpu_name (cosma::local_multiply_cpu).  A, B, C are the counter-based uniform matrices with seeds
 //       1, 2, 3 (element i of seed s = u(splitmix64(s * 0x9E3779B97F4A7C15 + i))),
 //       column-major with ld = m, k, m.  Writes C (m * n doubles).
 //   front_test multiply m n k steps alpha beta out.bin
@@ -214,6 +214,8 @@ int run_local_multiply(int argc, char **argv) {
     } else if (variant == "context") {
         auto ctx = cosma::make_context<double>(SIZE_MAX, 3, 64, 48, 80);   // small tiles: many streamed tiles
         cosma::local_multiply<double>(ctx, A.data(), B.data(), C.data(), m, n, k, alpha, beta, true);
+    } else if (variant == "cpu_name") {
+        cosma::local_multiply_cpu<double>(A.data(), B.data(), C.data(), m, n, k, alpha, beta);
     } else if (variant == "handle") {
         auto ctx = cosma::make_context<double>();
         cosma::local_multiply<double>(ctx->get_gpu_context(), A.data(), B.data(), C.data(), m, n, k, alpha, beta,

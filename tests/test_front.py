@@ -56,6 +56,7 @@ def test_prefixed_and_c_api_symbols_exported():
     demangled = subprocess.run(["nm", "-DC", "--defined-only", ca.LIB_PATH], capture_output=True, text=True,
                                check=True).stdout
     for sig in ("cosma::local_multiply<double>(cosma::cosma_context<double>*",
+                "cosma::local_multiply_cpu<double>(double*, double*, double*, int, int, int, double, double)",
                 "cosma::multiply<double>(cosma::CosmaMatrix<double>&",
                 "cosma::pxgemm<double>(char, char, int, int, int, double",
                 "cosma::get_context_instance<double>()",

@@ -1,4 +1,5 @@
-"""The C++ COSMA interface on the GPU: local_multiply (three overloads),
+"""The C++ COSMA interface on the GPU: local_multiply (three overloads and
+local_multiply_cpu, the reference's test-utility name),
 multiply with CosmaMatrix and sequential steps, the ScaLAPACK entry points
 (prefixed and interposed) and the C layout API, each against the
 double-double oracle.
@@ -39,7 +40,7 @@ def expected(ta, tb, m, n, k, alpha, A, B, beta, C):
     return ref
 
 
-@pytest.mark.parametrize("variant", ["singleton", "context", "handle"])
+@pytest.mark.parametrize("variant", ["singleton", "context", "handle", "cpu_name"])
 @pytest.mark.parametrize("m,n,k,alpha,beta", [(200, 150, 170, 1.0, 0.0), (333, 129, 257, -1.3, 0.5)])
 def test_local_multiply(variant, m, n, k, alpha, beta):
     got = drive("local_multiply", variant, m, n, k, alpha, beta).reshape((n, m)).T
