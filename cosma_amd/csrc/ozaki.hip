@@ -64,6 +64,7 @@ __global__ void clear_u64(unsigned long long *p, int64_t n) {
 // lane = r, wave = 64 k, one wave sum per k.  Workgroups cover STATS_KC * 64
 // (CONTIG) or 64 lanes x STATS_SPAN of the other dimension (4 waves x
 // STATS_SPAN / 4).
+// The contiguous form reads with non-temporal loads (8192^2: 140 -> 128 us).
 constexpr int STATS_SPAN = 256;
 constexpr int STATS_KC = 8;   // 64-wide k chunks per workgroup (contiguous vectors)
 
@@ -99,7 +100,7 @@ __global__ __launch_bounds__(256) void stats_kernel(const double *__restrict__ x
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     const int64_t r = rb + q0 + u;
-                    v[u] = (k < K && r < R) ? x[k + r * ld] : 0.0;
+                    v[u] = (k < K && r < R) ? __builtin_nontemporal_load(x + k + r * ld) : 0.0;
                 }
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
@@ -258,7 +259,9 @@ __global__ void split_contig(SplitArgs a, int64_t Rpad) {
     }
 }
 
-// op(X) rows strided (A 'N', B 'T'): 64 x 64 tiles transposed through LDS.
+// op(X) rows strided (A 'N', B 'T'): 64 x 64 tiles transposed through LDS
+// (non-temporal loads: 424 -> 406 us at 8192^2; for the contiguous split and
+// the recombination they were slower).
 // blockIdx.x -> 64 rows, blockIdx.y -> 64 k.
 __global__ __launch_bounds__(256) void split_strided(SplitArgs a) {
     __shared__ double tile[64][65];
@@ -268,7 +271,7 @@ __global__ __launch_bounds__(256) void split_strided(SplitArgs a) {
     for (int q = 0; q < 16; ++q) {
         const int kk = ty + 4 * q;
         const int64_t r = r0 + tx, k = k0 + kk;
-        tile[kk][tx] = (r < a.R && k < a.K) ? a.x[r + k * a.ld] : 0.0;
+        tile[kk][tx] = (r < a.R && k < a.K) ? __builtin_nontemporal_load(a.x + r + k * a.ld) : 0.0;
     }
     __syncthreads();
     const int row = threadIdx.x >> 2, kc = threadIdx.x & 3;
