@@ -44,7 +44,7 @@ PEAK_HBM = 8000.0       # GB/s, MI355X datasheet
 M = N = K = 16384
 ALPHA, BETA = 1.0, 0.5
 METRIC = "FP64-accurate TFLOP/s, 1 GPU: native DGEMM vs split-precision at stated error"
-LEGS = ["device", "host", "tall-skinny", "ozaki", "ozaki-target", "reorder", "batched", "miniapp"]
+LEGS = ["device", "host", "tall-skinny", "ozaki", "ozaki-8192", "ozaki-target", "reorder", "batched", "miniapp"]
 
 
 def parse():
@@ -55,8 +55,10 @@ def parse():
     p.add_argument("--workload", default=None, choices=LEGS,
                    help="run one leg alone and print its line: device (the headline with the split-precision "
                         "leg and the CPU baseline), host (16384^3 from pinned host memory), tall-skinny, ozaki "
-                        "(8192^3 phi sweep with the error of every count), ozaki-target (split precision at the "
+                        "(8192^3 phi sweep with the error of every count), ozaki-8192 (the 8192^3 moduli call "
+                        "alone at --phi, for profiling), ozaki-target (split precision at the "
                         "headline shape), reorder, batched, miniapp (2000^3 through cosma::multiply)")
+    p.add_argument("--phi", type=float, default=0.0, help="phi of the ozaki-8192 leg's operands")
     p.add_argument("--quick", action="store_true", help="headline, split precision and CPU baseline only")
     p.add_argument("--no-split", action="store_true", help="skip the split-precision summary leg")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
@@ -292,7 +294,8 @@ def ozaki_entry(info, t, t_plain, flops, err, n):
                          "traffic": None}}
 
 
-def split_precision(phis=(0.0, 1.0, 4.0), steps=5, warmup=1, n=8192, rows=(0, 4097), sweep=True):
+def split_precision(phis=(0.0, 1.0, 4.0), steps=5, warmup=1, n=8192, rows=(0, 4097), sweep=True,
+                    methods=("moduli", "moduli_bf16", "slices")):
     """8192^3 C = A*B (NN): plain FP64 MFMA DGEMM vs the split-precision DGEMM
     (moduli at the default 1e-14 tolerance with INT8 and with BF16 residues,
     slices at the default tolerance), inputs a = (u - 0.5) exp(phi z).  Errors
@@ -318,6 +321,8 @@ def split_precision(phis=(0.0, 1.0, 4.0), steps=5, warmup=1, n=8192, rows=(0, 40
                                       "rel_err": rows_error(C, rows, ref)}}
         for name, method in (("moduli", ca.OZAKI_MODULI), ("moduli_bf16", ca.OZAKI_MODULI_BF16),
                              ("slices", ca.OZAKI_SLICES)):
+            if name not in methods:
+                continue
             info = {}
 
             def oz():
@@ -696,6 +701,14 @@ def main():
         print(json.dumps(leg_line({"value": best, "unit": "TFLOP/s", "config": {
             "workload": "C=A*B m=n=k=8192, plain FP64 vs split precision, phi in {0,1,4}"}, "results": res},
             args.steps, args.warmup)), flush=True)
+        return
+    if w == "ozaki-8192":
+        res = split_precision(phis=(args.phi,), steps=args.steps, warmup=args.warmup, sweep=False, methods=("moduli",))
+        r = res[0]
+        print(json.dumps(leg_line({"value": r["moduli"]["tflops_fp64_equiv"], "unit": "TFLOP/s",
+                                   "ms_per_step": r["moduli"]["ms"], "config": {
+                                       "workload": f"C=A*B m=n=k=8192, moduli method at 1e-14, phi={args.phi}"},
+                                   "result": r}, args.steps, args.warmup)), flush=True)
         return
     if w in ("host", "tall-skinny"):
         print(json.dumps(leg_line(streamed(w, args.steps, args.warmup, not args.no_check), args.steps,
