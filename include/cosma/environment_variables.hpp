@@ -4,18 +4,21 @@
 // existing job scripts keep their meaning.  What each knob does here:
 //   COSMA_GPU_STREAMS             compute streams of the host-streaming GEMM
 //   COSMA_GPU_MAX_TILE_{M,N,K}    host-streaming tile sizes
-//   COSMA_GPU_MEMORY_PINNING      accepted; host memory is staged through
-//                                 pinned buffers (never registered in place)
+//   COSMA_GPU_MEMORY_PINNING      ON: the memory pool of CosmaMatrix is
+//                                 page-locked (hipHostMalloc); OFF: pageable,
+//                                 staged through the GEMM's pinned buffers
+//                                 (host memory is never registered in place)
 //   COSMA_CPU_MAX_MEMORY          memory limit in MB per rank: multiplies that
 //                                 need more (m*k + k*n + m*n elements) throw
 //   COSMA_DIM_THRESHOLD           pdgemm problems with min(m,n,k) below it are
 //                                 passed on to the next pdgemm (ScaLAPACK)
+//   COSMA_MEMORY_POOL_AMORTIZATION  growth factor of the memory pool's reserve
+//   COSMA_CPU_MEMORY_ALIGNMENT    byte alignment of the pool and its buffers
 //   COSMA_ADAPT_STRATEGY, COSMA_OVERLAP_COMM_AND_COMP,
-//   COSMA_MEMORY_POOL_AMORTIZATION, COSMA_MIN_LOCAL_DIMENSION,
-//   COSMA_CPU_MEMORY_ALIGNMENT, COSMA_GPU_UNIFIED_MEMORY
+//   COSMA_MIN_LOCAL_DIMENSION, COSMA_GPU_UNIFIED_MEMORY
 //                                 read and exposed; they steer multi-rank
-//                                 strategies and host pools that a single
-//                                 rank does not have.
+//                                 strategies and unified-memory GEMMs that a
+//                                 single rank on one MI355X does not have.
 #pragma once
 #include <cstddef>
 #include <limits>
