@@ -458,7 +458,9 @@ def batched(steps, warmup, count=4096, seed=5):
     """4096 independent C = A B problems (NN, alpha = 1, beta = 0) packed in
     three device buffers; one grouped persistent launch per step, replayed as
     a hipGraph after the plan is tuned; then the same batch in split-precision
-    form (grouped moduli plan at the 1e-14 tolerance) with its check."""
+    form (grouped moduli plan at the 1e-14 tolerance) with its check, with
+    the default split threshold (every problem here runs in FP64) and with
+    every problem split."""
     import numpy as np
     import torch
     import cosma_amd as ca
@@ -529,22 +531,25 @@ def batched(steps, warmup, count=4096, seed=5):
                         "note": "peak = sum flops / sum_i max(flops_i / 78.6 TF, bytes_i / 8 TB/s); "
                                 "frac uses the launch-inclusive event time of a graph replay"}}
     plan.close()
-    oz = ca.GroupedOzaki(h, probs)
-    oz.tune()
-    for _ in range(warmup):
-        oz.graph_run()
-    torch.cuda.synchronize()
-    s.record()
-    for _ in range(steps):
-        oz.graph_run()
-    e.record()
-    torch.cuda.synchronize()
-    oz_ms = s.elapsed_time(e) / steps
-    out["ozaki_form"] = {"ms_per_step": round(oz_ms, 4),
-                         "tflops_fp64_equiv": round(float(flops.sum()) / (oz_ms * 1e-3) / 1e12, 3),
-                         "vs_plain_fp64": round(ms / oz_ms, 3), "plan": oz.info(), "check": oz.check(1e-14),
-                         "max_rel_fro_sampled_problems_vs_double_double": sampled_err()}
-    oz.close()
+    # the split-precision form: problems below the split threshold (all of
+    # this batch) run on the FP64 grouped kernel; then every problem split
+    for key, split_min in (("ozaki_form", None), ("ozaki_form_split_all", 0)):
+        oz = ca.GroupedOzaki(h, probs, split_min_dim=split_min)
+        oz.tune()
+        for _ in range(warmup):
+            oz.graph_run()
+        torch.cuda.synchronize()
+        s.record()
+        for _ in range(steps):
+            oz.graph_run()
+        e.record()
+        torch.cuda.synchronize()
+        oz_ms = s.elapsed_time(e) / steps
+        out[key] = {"ms_per_step": round(oz_ms, 4),
+                    "tflops_fp64_equiv": round(float(flops.sum()) / (oz_ms * 1e-3) / 1e12, 3),
+                    "vs_plain_fp64": round(ms / oz_ms, 3), "plan": oz.info(), "check": oz.check(1e-14),
+                    "max_rel_fro_sampled_problems_vs_double_double": sampled_err()}
+        oz.close()
     h.close()
     del A, B, C
     torch.cuda.empty_cache()

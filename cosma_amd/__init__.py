@@ -39,6 +39,7 @@ OZAKI_MODULI = 0
 OZAKI_SLICES = 1
 OZAKI_MODULI_BF16 = 2
 OZAKI_STRICT = 0x100   # flag: the tolerance must hold for the rigorous bound
+OZAKI_SPLIT_MIN_DIM = 2048   # grouped split plans: smaller problems run in FP64 (COSMA_AMD_OZAKI_SPLIT_MIN_DIM)
 
 
 class Block(ctypes.Structure):
@@ -108,6 +109,9 @@ SIGNATURES = {
     "cosma_amd_grouped_destroy": (ctypes.c_int, [c_vp]),
     "cosma_amd_grouped_ozaki_create": (ctypes.c_int, [c_vp, c_vp, c_i64, ctypes.c_int, ctypes.c_double,
                                                       ctypes.POINTER(c_vp)]),
+    "cosma_amd_grouped_ozaki_create_ex": (ctypes.c_int, [c_vp, c_vp, c_i64, ctypes.c_int, ctypes.c_double, c_i64,
+                                                         ctypes.POINTER(c_vp)]),
+    "cosma_amd_grouped_ozaki_routing": (ctypes.c_int, [c_vp, ctypes.POINTER(c_i64), ctypes.POINTER(c_i64)]),
     "cosma_amd_grouped_ozaki_run": (ctypes.c_int, [c_vp, c_vp]),
     "cosma_amd_grouped_ozaki_graph_run": (ctypes.c_int, [c_vp, c_vp]),
     "cosma_amd_grouped_ozaki_tune": (ctypes.c_int, [c_vp, c_vp]),
@@ -300,17 +304,20 @@ class GroupedOzaki:
     """A grouped split-precision (moduli method) plan over device-resident problems.
 
     problems: iterable of (transa, transb, m, n, k, alpha, a_ptr, lda, b_ptr, ldb, beta, c_ptr, ldc);
-    moduli = 0 chooses the count from `tolerance` on the operands present now.
+    moduli = 0 chooses the count from `tolerance` on the operands present now.  Problems with
+    min(m, n, k) < split_min_dim are computed by the FP64 grouped kernel in the same run (None:
+    the library default, SPLIT_MIN_DIM; 0: every problem is split).
     """
 
-    def __init__(self, handle, problems, moduli=0, tolerance=1e-14):
+    def __init__(self, handle, problems, moduli=0, tolerance=1e-14, split_min_dim=None):
         probs = list(problems)
         arr = (GemmDesc * max(1, len(probs)))()
         for i, (ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc) in enumerate(probs):
             arr[i] = GemmDesc(_ch(ta), _ch(tb), m, n, k, alpha, a, lda, b, ldb, beta, c, ldc)
         p = c_vp()
-        check(lib().cosma_amd_grouped_ozaki_create(handle.ptr, ctypes.cast(arr, c_vp), len(probs), moduli,
-                                                   tolerance, ctypes.byref(p)))
+        split = OZAKI_SPLIT_MIN_DIM if split_min_dim is None else split_min_dim
+        check(lib().cosma_amd_grouped_ozaki_create_ex(handle.ptr, ctypes.cast(arr, c_vp), len(probs), moduli,
+                                                      tolerance, split, ctypes.byref(p)))
         self.ptr = p
 
     def run(self, stream=None):
@@ -329,7 +336,10 @@ class GroupedOzaki:
         mod, est, tiles, grid = ctypes.c_int(), ctypes.c_double(), c_i64(), ctypes.c_int()
         check(lib().cosma_amd_grouped_ozaki_info(self.ptr, ctypes.byref(mod), ctypes.byref(est), ctypes.byref(tiles),
                                                  ctypes.byref(grid)))
-        return {"moduli": mod.value, "error_estimate": est.value, "tiles": tiles.value, "grid": grid.value}
+        sp, fp = c_i64(), c_i64()
+        check(lib().cosma_amd_grouped_ozaki_routing(self.ptr, ctypes.byref(sp), ctypes.byref(fp)))
+        return {"moduli": mod.value, "error_estimate": est.value, "tiles": tiles.value, "grid": grid.value,
+                "split_problems": sp.value, "fp64_problems": fp.value}
 
     def check(self, tolerance=1e-14, stream=None):
         """Error check of the last run: worst expected error, its problem, moduli needed."""

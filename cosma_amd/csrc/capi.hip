@@ -336,11 +336,18 @@ int cosma_amd_grouped_destroy(cosma_amd_grouped_t plan) {
 
 int cosma_amd_grouped_ozaki_create(cosma_amd_handle_t h, const cosma_amd_gemm_desc *problems, int64_t count,
                                    int moduli, double tolerance, cosma_amd_grouped_ozaki_t *plan) {
+    return cosma_amd_grouped_ozaki_create_ex(h, problems, count, moduli, tolerance, COSMA_AMD_OZAKI_SPLIT_MIN_DIM,
+                                             plan);
+}
+
+int cosma_amd_grouped_ozaki_create_ex(cosma_amd_handle_t h, const cosma_amd_gemm_desc *problems, int64_t count,
+                                      int moduli, double tolerance, int64_t split_min_dim,
+                                      cosma_amd_grouped_ozaki_t *plan) {
     clear_error();
     CA_CHECK(h != nullptr, "grouped_ozaki_create: null handle");
     CA_CHECK(plan != nullptr, "grouped_ozaki_create: null plan pointer");
     CA_HIP(hipSetDevice(h->device));
-    return grouped_ozaki_create(h, problems, count, moduli, tolerance, plan);
+    return grouped_ozaki_create(h, problems, count, moduli, tolerance, split_min_dim, plan);
 }
 
 int cosma_amd_grouped_ozaki_run(cosma_amd_grouped_ozaki_t plan, void *stream) {
@@ -372,6 +379,14 @@ int cosma_amd_grouped_ozaki_info(cosma_amd_grouped_ozaki_t plan, int *moduli, do
     clear_error();
     CA_CHECK(plan != nullptr, "grouped_ozaki_info: null plan");
     grouped_ozaki_info(plan, moduli, error_estimate, tiles, grid);
+    return COSMA_AMD_OK;
+}
+
+int cosma_amd_grouped_ozaki_routing(cosma_amd_grouped_ozaki_t plan, int64_t *split_problems,
+                                    int64_t *fp64_problems) {
+    clear_error();
+    CA_CHECK(plan != nullptr, "grouped_ozaki_routing: null plan");
+    grouped_ozaki_routing(plan, split_problems, fp64_problems);
     return COSMA_AMD_OK;
 }
 

@@ -121,7 +121,8 @@ int launch_grouped(int cfg, const GroupedProblem *probs, int nprob, int64_t tota
 int grouped_create(cosma_amd_handle_t h, const cosma_amd_gemm_desc *d, int64_t count, int tile_config,
                    cosma_amd_grouped_t *out);
 int grouped_ozaki_create(cosma_amd_handle_t h, const cosma_amd_gemm_desc *d, int64_t count, int moduli,
-                         double tol, cosma_amd_grouped_ozaki_t *out);
+                         double tol, int64_t split_min, cosma_amd_grouped_ozaki_t *out);
+void grouped_ozaki_routing(cosma_amd_grouped_ozaki_t p, int64_t *split, int64_t *fp64);
 int grouped_ozaki_run(cosma_amd_grouped_ozaki_t p, hipStream_t s);
 int grouped_ozaki_graph_run(cosma_amd_grouped_ozaki_t p, hipStream_t s);
 int grouped_ozaki_tune(cosma_amd_grouped_ozaki_t p, hipStream_t s);

@@ -19,6 +19,14 @@
 // must meet the tolerance, doubled, for every problem.  Later runs reuse N on
 // whatever the operands hold; the exponents are recomputed on every run.
 //
+// Problems whose smallest dimension min(m, n, k) is below the plan's split
+// threshold (default COSMA_AMD_OZAKI_SPLIT_MIN_DIM = 2048) are not split:
+// the plan computes them with the FP64 MFMA grouped kernel of grouped.hip, in
+// the same run (and graph), because below that size the FP64 kernel is both
+// faster and FP64-accurate (its error bound is the FP64 GEMM's, k u |A||B|):
+// the 4096-problem batch of dimensions 16..512 took 5.4 ms split at 14
+// moduli against 0.88 ms in FP64.  A threshold of 0 splits every problem.
+//
 // Layout of the workspace (one allocation owned by the plan): per problem the
 // residues of op(A) (N planes of Mp x Kp bytes, row r of op(A) k-contiguous),
 // of op(B) (N planes of Np x Kp, column j k-contiguous), and of the products
@@ -71,7 +79,9 @@ struct GzModuli {
 
 struct cosma_amd_grouped_ozaki_s {
     cosma_amd_handle_t h = nullptr;
-    int nprob = 0;
+    int nprob = 0;                         // problems computed by the split method
+    cosma_amd_grouped_t fp64 = nullptr;    // the problems below the split threshold (FP64 grouped plan)
+    int64_t nfp64 = 0;
     int moduli = 0;
     double estimate = 0.0;
     int64_t total_tiles = 0;
@@ -610,6 +620,7 @@ int grouped_ozaki_destroy(cosma_amd_grouped_ozaki_t p) {
     (void)hipSetDevice(p->h->device);
     drop_graph(p);
     if (p->capture) (void)hipStreamDestroy(p->capture);
+    if (p->fp64) grouped_destroy(p->fp64);
     if (p->dev) (void)hipFree(p->dev);
     if (p->flags) (void)hipFree(p->flags);
     if (p->snorm) (void)hipFree(p->snorm);
@@ -623,8 +634,12 @@ int grouped_ozaki_destroy(cosma_amd_grouped_ozaki_t p) {
 }
 
 int grouped_ozaki_run(cosma_amd_grouped_ozaki_t p, hipStream_t s) {
-    if (p->nprob == 0) return COSMA_AMD_OK;
     CA_HIP(hipSetDevice(p->h->device));
+    if (p->fp64) {
+        const int rc = grouped_run(p->fp64, s);
+        if (rc) return rc;
+    }
+    if (p->nprob == 0) return COSMA_AMD_OK;
     const dim3 gv((unsigned)p->nprob, (unsigned)(2 * p->vgroups));
     const dim3 g1((unsigned)((p->nprob + 255) / 256));
     hipLaunchKernelGGL(gz_clear, g1, dim3(256), 0, s, p->flags, p->snorm, p->emax, p->xnorm, p->nprob);
@@ -646,7 +661,7 @@ int grouped_ozaki_run(cosma_amd_grouped_ozaki_t p, hipStream_t s) {
 }
 
 int grouped_ozaki_graph_run(cosma_amd_grouped_ozaki_t p, hipStream_t s) {
-    if (p->nprob == 0) return COSMA_AMD_OK;
+    if (p->nprob == 0 && !p->fp64) return COSMA_AMD_OK;
     CA_HIP(hipSetDevice(p->h->device));
     if (!p->exec) {
         if (!p->capture) CA_HIP(hipStreamCreateWithFlags(&p->capture, hipStreamNonBlocking));
@@ -667,8 +682,13 @@ int grouped_ozaki_graph_run(cosma_amd_grouped_ozaki_t p, hipStream_t s) {
 }
 
 int grouped_ozaki_tune(cosma_amd_grouped_ozaki_t p, hipStream_t s) {
-    if (p->nprob == 0 || p->total_tiles == 0) return COSMA_AMD_OK;
     CA_HIP(hipSetDevice(p->h->device));
+    if (p->fp64) {
+        const int rc = grouped_tune(p->fp64, s);
+        drop_graph(p);
+        if (rc) return rc;
+    }
+    if (p->nprob == 0 || p->total_tiles == 0) return COSMA_AMD_OK;
     hipEvent_t a, b;
     CA_HIP(hipEventCreate(&a));
     CA_HIP(hipEventCreate(&b));
@@ -702,8 +722,12 @@ int grouped_ozaki_tune(cosma_amd_grouped_ozaki_t p, hipStream_t s) {
 }
 
 int grouped_ozaki_create(cosma_amd_handle_t h, const cosma_amd_gemm_desc *d, int64_t count, int moduli,
-                         double tol, cosma_amd_grouped_ozaki_t *out) {
+                         double tol, int64_t split_min, cosma_amd_grouped_ozaki_t *out) {
     *out = nullptr;
+    if (split_min < 0) {
+        set_error("grouped ozaki: split threshold %lld is negative", (long long)split_min);
+        return COSMA_AMD_ERR_INVALID_ARG;
+    }
     if (count < 0 || count > 0x7fffffff || (count > 0 && !d)) {
         set_error("grouped ozaki: invalid problem list");
         return COSMA_AMD_ERR_INVALID_ARG;
@@ -717,6 +741,7 @@ int grouped_ozaki_create(cosma_amd_handle_t h, const cosma_amd_gemm_desc *d, int
     std::vector<OzProblem> probs;
     std::vector<double> cost;
     std::vector<int64_t> orig;
+    std::vector<cosma_amd_gemm_desc> plain;   // below the split threshold: FP64 grouped kernel
     for (int64_t i = 0; i < count; ++i) {
         const cosma_amd_gemm_desc &q = d[i];
         const char ta = norm_trans(q.transa), tb = norm_trans(q.transb);
@@ -733,6 +758,10 @@ int grouped_ozaki_create(cosma_amd_handle_t h, const cosma_amd_gemm_desc *d, int
             set_error("grouped ozaki: problem %lld has a leading dimension too small or an operand that is not "
                       "device memory", (long long)i);
             return COSMA_AMD_ERR_INVALID_ARG;
+        }
+        if (std::min(q.m, std::min(q.n, q.k)) < split_min) {
+            plain.push_back(q);
+            continue;
         }
         if (!scale_only && (double)q.k * 16384.0 >= 2147483648.0) {
             set_error("grouped ozaki: problem %lld: k = %lld too large for exact INT32 accumulation", (long long)i,
@@ -774,15 +803,20 @@ int grouped_ozaki_create(cosma_amd_handle_t h, const cosma_amd_gemm_desc *d, int
     p->nprob = (int)sorted.size();
     p->grid = 1;
     for (int i : order) p->original.push_back(orig[i]);
-    if (p->nprob == 0) {
-        p->moduli = moduli ? moduli : 2;
-        *out = p;
-        return COSMA_AMD_OK;
-    }
     auto fail = [&](int rc) {
         grouped_ozaki_destroy(p);
         return rc;
     };
+    if (!plain.empty()) {
+        const int rc = grouped_create(h, plain.data(), (int64_t)plain.size(), 0, &p->fp64);
+        if (rc) return fail(rc);
+        p->nfp64 = (int64_t)plain.size();
+    }
+    if (p->nprob == 0) {
+        p->moduli = moduli;   // 0: no problem is split, no count was chosen
+        *out = p;
+        return COSMA_AMD_OK;
+    }
     // vector groups of 64 per operand: the y extent of the exponent and split grids
     int64_t vg = 1;
     for (const auto &g : sorted) vg = std::max(vg, std::max(g.Mp, g.Np) / 64);
@@ -1006,6 +1040,11 @@ void grouped_ozaki_info(cosma_amd_grouped_ozaki_t p, int *moduli, double *estima
     if (estimate) *estimate = p->estimate;
     if (tiles) *tiles = p->total_tiles;
     if (grid) *grid = p->grid;
+}
+
+void grouped_ozaki_routing(cosma_amd_grouped_ozaki_t p, int64_t *split, int64_t *fp64) {
+    if (split) *split = p->nprob;
+    if (fp64) *fp64 = p->nfp64;
 }
 
 int grouped_ozaki_set_grid(cosma_amd_grouped_ozaki_t p, int grid) {

@@ -317,6 +317,13 @@ int cosma_amd_grouped_destroy(cosma_amd_grouped_t plan);
 typedef struct cosma_amd_grouped_ozaki_s *cosma_amd_grouped_ozaki_t;
 
 /*
+ * Problems of a grouped split-precision plan whose smallest dimension
+ * min(m, n, k) is below this are computed by the FP64 MFMA grouped kernel
+ * instead (see cosma_amd_grouped_ozaki_create_ex).
+ */
+#define COSMA_AMD_OZAKI_SPLIT_MIN_DIM 2048
+
+/*
  * Plans the split-precision (moduli method, as cosma_amd_dgemm_ozaki) form of
  * a grouped GEMM over `count` independent device-resident problems: per run,
  * four grouped launches (exponents, residues, the INT8 products of every
@@ -341,6 +348,22 @@ typedef struct cosma_amd_grouped_ozaki_s *cosma_amd_grouped_ozaki_t;
 int cosma_amd_grouped_ozaki_create(cosma_amd_handle_t handle, const cosma_amd_gemm_desc *problems,
                                    int64_t count, int moduli, double tolerance,
                                    cosma_amd_grouped_ozaki_t *plan);
+/*
+ * The same with an explicit split threshold: problems with min(m, n, k) <
+ * split_min_dim are computed in the same run (and graph) by the FP64 MFMA
+ * grouped kernel of cosma_amd_grouped_create, which below that size is
+ * faster than splitting and FP64-accurate (the FP64 GEMM's error bound); the
+ * rest by the split method.  cosma_amd_grouped_ozaki_create uses
+ * COSMA_AMD_OZAKI_SPLIT_MIN_DIM; 0 splits every problem.  The moduli count,
+ * its estimate and cosma_amd_grouped_ozaki_check cover the split problems
+ * (a count of 0 is reported when none is split and none was given).
+ */
+int cosma_amd_grouped_ozaki_create_ex(cosma_amd_handle_t handle, const cosma_amd_gemm_desc *problems,
+                                      int64_t count, int moduli, double tolerance, int64_t split_min_dim,
+                                      cosma_amd_grouped_ozaki_t *plan);
+/* Number of problems of the plan computed by the split method and in FP64. */
+int cosma_amd_grouped_ozaki_routing(cosma_amd_grouped_ozaki_t plan, int64_t *split_problems,
+                                    int64_t *fp64_problems);
 int cosma_amd_grouped_ozaki_run(cosma_amd_grouped_ozaki_t plan, void *stream);
 int cosma_amd_grouped_ozaki_graph_run(cosma_amd_grouped_ozaki_t plan, void *stream);
 int cosma_amd_grouped_ozaki_tune(cosma_amd_grouped_ozaki_t plan, void *stream);

@@ -79,7 +79,7 @@ def test_python_constants_match_the_header():
     # The enum values the Python binding passes through the ABI are the ones
     # the header declares (compiled and printed by the C compiler).
     names = ["COSMA_AMD_OZAKI_MODULI", "COSMA_AMD_OZAKI_SLICES", "COSMA_AMD_OZAKI_MODULI_BF16",
-             "COSMA_AMD_OZAKI_STRICT"]
+             "COSMA_AMD_OZAKI_STRICT", "COSMA_AMD_OZAKI_SPLIT_MIN_DIM"]
     src = "#include <stdio.h>\n#include \"cosma_amd.h\"\nint main(void){" + "".join(
         f'printf("%d\\n", (int){n});' for n in names) + "return 0;}\n"
     import tempfile
@@ -89,7 +89,8 @@ def test_python_constants_match_the_header():
                            input=src, capture_output=True, text=True)
         assert r.returncode == 0, r.stderr
         values = [int(v) for v in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
-    assert values == [ca.OZAKI_MODULI, ca.OZAKI_SLICES, ca.OZAKI_MODULI_BF16, ca.OZAKI_STRICT]
+    assert values == [ca.OZAKI_MODULI, ca.OZAKI_SLICES, ca.OZAKI_MODULI_BF16, ca.OZAKI_STRICT,
+                      ca.OZAKI_SPLIT_MIN_DIM]
 
 
 def test_info_struct_layout_matches_the_header():

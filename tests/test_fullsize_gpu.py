@@ -254,12 +254,20 @@ def test_batch_4096_plain_grouped(batch):
         assert e <= 4 * int(batch["dims"][i, 2]) * U, (i, e)
 
 
-def test_batch_4096_split_precision_grouped(batch):
+@pytest.mark.parametrize("split_min_dim", [None, 0])
+def test_batch_4096_split_precision_grouped(batch, split_min_dim):
+    """The split-precision plan of the batch: with the default threshold every
+    problem (dimensions <= 512) runs on the FP64 grouped kernel; with a
+    threshold of 0 every problem is split."""
     import torch
     import cosma_amd as ca
     h = ca.Handle(0)
-    say("grouped split-precision plan over the 4096-problem batch at 1e-14: tune, graph replay")
-    plan = ca.GroupedOzaki(h, batch["probs"], tolerance=1e-14)
+    say(f"grouped split-precision plan over the 4096-problem batch at 1e-14, split_min_dim={split_min_dim}: "
+        f"tune, graph replay")
+    plan = ca.GroupedOzaki(h, batch["probs"], tolerance=1e-14, split_min_dim=split_min_dim)
+    info = plan.info()
+    n = len(batch["probs"])
+    assert (info["split_problems"], info["fp64_problems"]) == ((0, n) if split_min_dim is None else (n, 0)), info
     plan.tune()
     batch["C"].fill_(float("nan"))
     plan.graph_run()

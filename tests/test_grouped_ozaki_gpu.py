@@ -1,6 +1,9 @@
 """Grouped split-precision (moduli method) plans through the C ABI.
 
-With the same moduli count, each problem of a plan must come out
+Problems below a plan's split threshold (min(m, n, k) < 2048 by default) run
+on the FP64 grouped kernel and must come out bit-identical to cosma_amd_dgemm
+on them; the tests of the split method force it with a threshold of 0.
+With the same moduli count, each split problem of a plan must come out
 bit-identical to cosma_amd_dgemm_ozaki on it (same exponents, same P, exact
 residue products and recombination), for the direct run and for every replay
 of the captured hipGraph.  Problems are also checked against the
@@ -80,6 +83,19 @@ def single_results(handle, probs, moduli):
     return [from_dev(c, p["m"]) for c, p in zip(out, probs)]
 
 
+def single_dgemm_results(handle, probs):
+    """The same problems through the plain FP64 cosma_amd_dgemm."""
+    import torch
+    out = []
+    for p in probs:
+        c = p["dC"].clone()
+        handle.dgemm(p["ta"], p["tb"], p["m"], p["n"], p["k"], p["alpha"], p["dA"].data_ptr(), p["lda"],
+                     p["dB"].data_ptr(), p["ldb"], p["beta"], c.data_ptr(), p["ldc"])
+        out.append(c)
+    torch.cuda.synchronize()
+    return [from_dev(c, p["m"]) for c, p in zip(out, probs)]
+
+
 def same(got, exp):
     return all(np.array_equal(g, e, equal_nan=True) for g, e in zip(got, exp))
 
@@ -103,7 +119,7 @@ def test_bit_identical_to_single_calls(handle, problems, moduli):
     import cosma_amd as ca
     reset_c(problems)
     expected = single_results(handle, problems, moduli)
-    plan = ca.GroupedOzaki(handle, descs(problems), moduli=moduli)
+    plan = ca.GroupedOzaki(handle, descs(problems), moduli=moduli, split_min_dim=0)
     assert plan.info()["moduli"] == moduli
     plan.run()
     torch.cuda.synchronize()
@@ -116,7 +132,7 @@ def test_default_tolerance_against_oracle(handle):
     import cosma_amd as ca
     probs = make_problems(40, 9, max_dim=300, phi=1.0)
     reset_c(probs)
-    plan = ca.GroupedOzaki(handle, descs(probs))
+    plan = ca.GroupedOzaki(handle, descs(probs), split_min_dim=0)
     info = plan.info()
     assert 2 <= info["moduli"] <= 20 and info["error_estimate"] <= 1e-14
     plan.run()
@@ -145,7 +161,7 @@ def test_check_after_operands_change(handle):
     probs = make_problems(12, 31, max_dim=120)
     probs = [p for p in probs if p["k"] >= 4 and p["alpha"] != 0.0 and p["A"].any()]
     reset_c(probs)
-    plan = ca.GroupedOzaki(handle, descs(probs))
+    plan = ca.GroupedOzaki(handle, descs(probs), split_min_dim=0)
     base = plan.info()["moduli"]
     plan.run()
     torch.cuda.synchronize()
@@ -177,7 +193,7 @@ def test_check_after_operands_change(handle):
     assert 0 <= chk["worst_problem"] < len(probs)
     plan.close()
     assert chk["moduli_needed"] <= 20
-    better = ca.GroupedOzaki(handle, descs(probs), moduli=chk["moduli_needed"])
+    better = ca.GroupedOzaki(handle, descs(probs), moduli=chk["moduli_needed"], split_min_dim=0)
     reset_c(probs)
     better.run()
     torch.cuda.synchronize()
@@ -223,7 +239,7 @@ def test_norm_bound_bits_match_single_calls(handle):
     reset_c(probs)
     torch.cuda.synchronize()
     expected = single_results(handle, probs, 8)
-    plan = ca.GroupedOzaki(handle, descs(probs), moduli=8)
+    plan = ca.GroupedOzaki(handle, descs(probs), moduli=8, split_min_dim=0)
     plan.graph_run()
     torch.cuda.synchronize()
     assert same(results(probs), expected)
@@ -247,7 +263,7 @@ def test_graph_replays_match_eager(handle, problems):
     import torch
     import cosma_amd as ca
     reset_c(problems)
-    plan = ca.GroupedOzaki(handle, descs(problems), moduli=12)
+    plan = ca.GroupedOzaki(handle, descs(problems), moduli=12, split_min_dim=0)
     plan.run()
     torch.cuda.synchronize()
     expected = results(problems)
@@ -264,7 +280,7 @@ def test_tuner_keeps_results(handle, problems):
     import torch
     import cosma_amd as ca
     reset_c(problems)
-    plan = ca.GroupedOzaki(handle, descs(problems), moduli=10)
+    plan = ca.GroupedOzaki(handle, descs(problems), moduli=10, split_min_dim=0)
     plan.run()
     torch.cuda.synchronize()
     expected = results(problems)
@@ -295,10 +311,10 @@ def test_non_finite_operands_take_fp64(handle):
     bad["dA"], bad["lda"] = to_dev(bad["A"], bad["lda"])
     reset_c(probs)
     # the non-finite problem does not use the moduli: it does not drive the count
-    auto = ca.GroupedOzaki(handle, descs(probs))
+    auto = ca.GroupedOzaki(handle, descs(probs), split_min_dim=0)
     assert auto.info()["moduli"] < 20 and np.isfinite(auto.info()["error_estimate"])
     auto.close()
-    plan = ca.GroupedOzaki(handle, descs(probs), moduli=12)
+    plan = ca.GroupedOzaki(handle, descs(probs), moduli=12, split_min_dim=0)
     plan.run()
     torch.cuda.synchronize()
     got = results(probs)
@@ -327,3 +343,61 @@ def test_invalid_plans_are_rejected(handle):
     empty.run()
     empty.graph_run()
     empty.close()
+
+
+def test_small_problems_run_in_fp64_by_default(handle):
+    """With the default threshold every problem of dimensions below 2048 runs
+    on the FP64 grouped kernel: bit-identical to cosma_amd_dgemm on it, for
+    the direct run and for graph replays, with no moduli count chosen."""
+    import torch
+    import cosma_amd as ca
+    probs = make_problems(40, 21, max_dim=300)
+    reset_c(probs)
+    expected = single_dgemm_results(handle, probs)
+    plan = ca.GroupedOzaki(handle, descs(probs))
+    info = plan.info()
+    assert info["fp64_problems"] == len(probs) and info["split_problems"] == 0, info
+    assert info["moduli"] == 0 and info["tiles"] == 0
+    plan.tune()
+    reset_c(probs)
+    plan.run()
+    torch.cuda.synchronize()
+    assert same(results(probs), expected)
+    for _ in range(3):
+        reset_c(probs)
+        plan.graph_run()
+        torch.cuda.synchronize()
+        assert same(results(probs), expected)
+    assert plan.check()["worst_problem"] == -1
+    plan.close()
+
+
+def test_mixed_routing_at_a_threshold(handle):
+    """A threshold inside the batch: problems with min(m, n, k) >= 64 are split
+    (bit-identical to cosma_amd_dgemm_ozaki with the plan's count), the others
+    run in FP64 (bit-identical to cosma_amd_dgemm), in one run and one graph."""
+    import torch
+    import cosma_amd as ca
+    probs = make_problems(40, 22, max_dim=160)
+    threshold, moduli = 64, 14
+    split = [min(p["m"], p["n"], p["k"]) >= threshold for p in probs]
+    assert 0 < sum(split) < len(probs)
+    reset_c(probs)
+    exp_oz = single_results(handle, probs, moduli)
+    exp_fp = single_dgemm_results(handle, probs)
+    expected = [o if s else f for o, f, s in zip(exp_oz, exp_fp, split)]
+    plan = ca.GroupedOzaki(handle, descs(probs), moduli=moduli, split_min_dim=threshold)
+    info = plan.info()
+    assert info["split_problems"] == sum(split) and info["fp64_problems"] == len(probs) - sum(split), info
+    assert info["moduli"] == moduli
+    reset_c(probs)
+    plan.run()
+    torch.cuda.synchronize()
+    assert same(results(probs), expected)
+    for _ in range(2):
+        reset_c(probs)
+        plan.graph_run()
+        torch.cuda.synchronize()
+        assert same(results(probs), expected)
+    plan.close()
+
