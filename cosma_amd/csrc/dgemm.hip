@@ -17,10 +17,12 @@
 //     Rows are stored XOR-swizzled, tile[kp][row ^ kp ^ (bit 4 of row)] (flips the low
 //     three row bits), with no padding: the MFMA reads (16 lanes on 16
 //     consecutive rows of one kp) and the k-contiguous stores (eight lanes
-//     per row, one per kp) both hit distinct bank groups.  Measured with
-//     SQ_LDS_BANK_CONFLICT on gfx950: unpadded + swizzled reads and stores
-//     are conflict-free, while a one-entry row pad (the classic fix) costs
-//     0.76 conflict cycles per fragment read.
+//     per row, one per kp) are meant to hit distinct bank groups.  Measured
+//     with SQ_LDS_BANK_CONFLICT on gfx950: a probe of the fragment reads
+//     alone found them conflict-free (a one-entry row pad, the classic fix,
+//     costs 0.76 conflict cycles per read); the whole TN 16384^3 kernel
+//     still counts 0.89 conflict cycles per LDS instruction, 1.4% of its
+//     cycles.
 //     The order in which k is summed is a permutation of 0..K-1; it only
 //     changes rounding, not the mathematical result.
 //   * LDS is double-buffered: the global loads of step t+1 are issued into
