@@ -109,3 +109,20 @@ def test_info_struct_layout_matches_the_header():
         values = [int(v) for v in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
     assert values[0] == ctypes.sizeof(ca.OzakiInfo)
     assert values[1:] == [getattr(ca.OzakiInfo, f).offset for f in fields]
+
+
+# The compiler build the device code was validated with.  i8gemm.hip's BF16
+# residue reduction reads and writes the accumulators in inline assembly and
+# places the wait states around it by hand (the compiler's hazard recognizer
+# does not look into asm blocks); its results were checked bit-identical to
+# the INT8 path on the GPU (tests/test_ozaki_gpu.py, BF16 cases) with this
+# compiler only.
+VALIDATED_COMPILER = "roc-7.2.0 26014"
+
+
+def test_device_compiler_is_the_validated_one():
+    from cosma_amd import build as cbuild
+    out = subprocess.run([cbuild.HIPCC, "--version"], capture_output=True, text=True).stdout
+    assert VALIDATED_COMPILER in out, (
+        "hipcc differs from the compiler the BF16 reduction's hand-placed wait states were validated with; "
+        "re-run the BF16 split-precision GPU tests before trusting this build:\n" + out)
