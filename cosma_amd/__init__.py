@@ -122,6 +122,7 @@ SIGNATURES = {
                                                      ctypes.POINTER(c_i64), ctypes.POINTER(ctypes.c_int), c_vp]),
     "cosma_amd_grouped_ozaki_destroy": (ctypes.c_int, [c_vp]),
     "cosma_amd_host_alloc": (ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(c_vp)]),
+    "cosma_amd_test_fail_allocations": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "cosma_amd_host_free": (ctypes.c_int, [c_vp]),
     "cosma_amd_dgemm_layout": (ctypes.c_int, [c_vp, c_char, c_char, c_i64, c_i64, c_i64, c_dbl,
                                               c_vp, c_i64, c_vp, c_i64, c_dbl, c_vp, c_i64]),
@@ -368,6 +369,12 @@ def fill_uniform(ptr, rows, cols, ld, seed, offset=0, stream=None):
 def fill_lognormal(ptr, rows, cols, ld, seed, phi, stream=None):
     """a = (u - 0.5) * exp(phi * z) fill of a column-major device matrix."""
     check(lib().cosma_amd_fill_lognormal(ptr, rows, cols, ld, seed, phi, stream))
+
+
+def fail_allocations(skip, count):
+    """Lets the library's next `skip` device or page-locked allocations through and fails the
+    `count` after them (count 0 disarms); for tests of the error paths."""
+    check(lib().cosma_amd_test_fail_allocations(skip, count))
 
 
 def set_dgemm_tuning(tile_config=0, group_m=0):

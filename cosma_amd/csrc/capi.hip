@@ -1,5 +1,7 @@
-// C ABI glue: argument validation, handle lifetime, error text.
+// C ABI glue: argument validation, handle lifetime, error text, and the
+// library's allocation wrappers.
 #include <cstdarg>
+#include <mutex>
 #include <cstring>
 
 #include "common.hpp"
@@ -30,6 +32,38 @@ static hipStream_t pick_stream(cosma_amd_handle_t, void *stream) {
 
 // Checks the usual BLAS leading-dimension rule ld >= max(1, rows).
 static bool ld_ok(int64_t ld, int64_t rows) { return ld >= (rows > 1 ? rows : 1); }
+
+// Failure injection (cosma_amd_test_fail_allocations): allocations to let
+// through first, then allocations to fail.
+static std::mutex g_fail_mu;
+static int g_fail_skip = 0, g_fail_count = 0;
+
+static bool fail_this_allocation() {
+    std::lock_guard<std::mutex> lock(g_fail_mu);
+    if (g_fail_count == 0) return false;
+    if (g_fail_skip > 0) {
+        --g_fail_skip;
+        return false;
+    }
+    --g_fail_count;
+    return true;
+}
+
+hipError_t dev_alloc(void **p, size_t bytes) {
+    if (fail_this_allocation()) {
+        *p = nullptr;
+        return hipErrorOutOfMemory;
+    }
+    return hipMalloc(p, bytes);
+}
+
+hipError_t pinned_alloc(void **p, size_t bytes) {
+    if (fail_this_allocation()) {
+        *p = nullptr;
+        return hipErrorOutOfMemory;
+    }
+    return hipHostMalloc(p, bytes, hipHostMallocDefault);
+}
 
 }  // namespace cosma_amd
 
@@ -410,7 +444,7 @@ int cosma_amd_host_alloc(size_t bytes, void **ptr) {
     CA_CHECK(ptr != nullptr, "host_alloc: null output pointer");
     *ptr = nullptr;
     if (bytes == 0) return COSMA_AMD_OK;
-    hipError_t e = hipHostMalloc(ptr, bytes, hipHostMallocDefault);
+    hipError_t e = pinned_alloc(ptr, bytes);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         *ptr = nullptr;
@@ -444,6 +478,14 @@ int cosma_amd_fill_lognormal(double *x, int64_t rows, int64_t cols, int64_t ld, 
     CA_CHECK(x != nullptr, "fill_lognormal: null pointer");
     CA_CHECK(ld_ok(ld, rows), "fill_lognormal: ld=%lld < rows=%lld", (long long)ld, (long long)rows);
     return launch_fill_lognormal(x, rows, cols, ld, seed, phi, static_cast<hipStream_t>(stream));
+}
+
+int cosma_amd_test_fail_allocations(int skip, int count) {
+    clear_error();
+    std::lock_guard<std::mutex> lock(g_fail_mu);
+    g_fail_skip = skip > 0 ? skip : 0;
+    g_fail_count = count > 0 ? count : 0;
+    return COSMA_AMD_OK;
 }
 
 }  // extern "C"

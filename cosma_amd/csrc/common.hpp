@@ -19,6 +19,11 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // Thread-local error text backing cosma_amd_last_error().
 void set_error(const char *fmt, ...);
+// Every device (hipMalloc) and page-locked host (hipHostMalloc) allocation of
+// the library goes through these, so that tests can make the next ones fail
+// (cosma_amd_test_fail_allocations).
+hipError_t dev_alloc(void **p, size_t bytes);
+hipError_t pinned_alloc(void **p, size_t bytes);
 void clear_error();
 
 // Records a HIP failure and returns COSMA_AMD_ERR_HIP.

@@ -153,7 +153,7 @@ int grouped_create(cosma_amd_handle_t h, const cosma_amd_gemm_desc *d, int64_t c
         std::vector<int64_t> starts;
         for (const GroupedProblem &g : sorted) starts.push_back(g.tile_start);
         const size_t dbytes = sorted.size() * sizeof(GroupedProblem), sbytes = starts.size() * sizeof(int64_t);
-        hipError_t e = hipMalloc(&p->dev[cfg], dbytes + sbytes);
+        hipError_t e = dev_alloc(reinterpret_cast<void **>(&p->dev[cfg]), dbytes + sbytes);
         if (e != hipSuccess) {
             (void)hipGetLastError();
             set_error("grouped dgemm: descriptor allocation failed (%s)", hipGetErrorString(e));

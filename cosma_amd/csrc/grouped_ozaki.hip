@@ -827,7 +827,7 @@ int grouped_ozaki_create(cosma_amd_handle_t h, const cosma_amd_gemm_desc *d, int
     }
     p->vgroups = (int)vg;
     auto alloc = [&](void **ptr, size_t bytes) -> int {
-        const hipError_t e = hipMalloc(ptr, std::max<size_t>(bytes, 256));
+        const hipError_t e = dev_alloc(ptr, std::max<size_t>(bytes, 256));
         if (e != hipSuccess) {
             (void)hipGetLastError();
             *ptr = nullptr;
@@ -973,7 +973,7 @@ int grouped_ozaki_check(cosma_amd_grouped_ozaki_t p, double tol, double *worst, 
     CA_HIP(hipSetDevice(p->h->device));
     const size_t n = (size_t)p->nprob;
     double *dg = nullptr;
-    if (hipMalloc(reinterpret_cast<void **>(&dg), 2 * n * sizeof(double)) != hipSuccess) {
+    if (dev_alloc(reinterpret_cast<void **>(&dg), 2 * n * sizeof(double)) != hipSuccess) {
         (void)hipGetLastError();
         set_error("grouped ozaki: check: allocation failed");
         return COSMA_AMD_ERR_ALLOC;

@@ -672,7 +672,7 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
         if (buf.ptr) CA_HIP(hipFree(buf.ptr));
         buf.ptr = nullptr;
         buf.bytes = 0;
-        hipError_t e = hipMalloc(&buf.ptr, need);
+        hipError_t e = dev_alloc(&buf.ptr, need);
         if (e != hipSuccess) {
             (void)hipGetLastError();
             set_error("ozaki: workspace allocation of %zu bytes failed", need);
@@ -734,7 +734,7 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
             if (h->ozaki_host.ptr) CA_HIP(hipHostFree(h->ozaki_host.ptr));
             h->ozaki_host.ptr = nullptr;
             h->ozaki_host.bytes = 0;
-            const hipError_t e = hipHostMalloc(&h->ozaki_host.ptr, host_bytes, hipHostMallocDefault);
+            const hipError_t e = pinned_alloc(&h->ozaki_host.ptr, host_bytes);
             if (e != hipSuccess) {
                 (void)hipGetLastError();
                 set_error("ozaki: page-locked statistics buffer of %zu bytes failed", host_bytes);

@@ -90,9 +90,10 @@ int grow(DevBuf &b, size_t bytes) {
         b.ptr = nullptr;
         b.bytes = 0;
     }
-    hipError_t e = hipMalloc(&b.ptr, bytes);
+    hipError_t e = dev_alloc(&b.ptr, bytes);
     if (e != hipSuccess) {
         (void)hipGetLastError();
+        b.ptr = nullptr;
         set_error("host streaming: hipMalloc of %zu bytes failed (%s)", bytes, hipGetErrorString(e));
         return COSMA_AMD_ERR_ALLOC;
     }
@@ -119,16 +120,31 @@ int record(cosma_amd_handle_t h, hipStream_t s, hipEvent_t *ev) {
 
 int ensure_slots(std::vector<StagingSlot> &slots) {
     if (!slots.empty()) return COSMA_AMD_OK;
+    // all N_SLOTS or none: a failure releases the slots made so far, so a
+    // later call starts over with a full ring
+    auto release = [&]() {
+        for (auto &s : slots) {
+            if (s.ev) (void)hipEventDestroy(s.ev);
+            if (s.ptr) (void)hipHostFree(s.ptr);
+        }
+        slots.clear();
+    };
     for (int i = 0; i < N_SLOTS; ++i) {
         StagingSlot s;
-        hipError_t e = hipHostMalloc(&s.ptr, SLOT_BYTES, hipHostMallocDefault);
+        hipError_t e = pinned_alloc(&s.ptr, SLOT_BYTES);
         if (e != hipSuccess) {
             (void)hipGetLastError();
+            release();
             set_error("host streaming: pinned staging allocation failed (%s)", hipGetErrorString(e));
             return COSMA_AMD_ERR_ALLOC;
         }
         s.bytes = SLOT_BYTES;
-        CA_HIP(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming));
+        e = hipEventCreateWithFlags(&s.ev, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            (void)hipHostFree(s.ptr);
+            release();
+            return hip_fail(e, "hipEventCreateWithFlags", __FILE__, __LINE__);
+        }
         slots.push_back(s);
     }
     return COSMA_AMD_OK;

@@ -387,6 +387,18 @@ int cosma_amd_grouped_ozaki_check(cosma_amd_grouped_ozaki_t plan, double toleran
                                   int64_t *worst_problem, int *moduli_needed, void *stream);
 int cosma_amd_grouped_ozaki_destroy(cosma_amd_grouped_ozaki_t plan);
 
+/* ------------------------------------------------------ failure injection */
+/*
+ * Lets the next `skip` device (hipMalloc) or page-locked host (hipHostMalloc)
+ * allocations of the library through and makes the `count` after them fail
+ * as if memory were exhausted (count 0 disarms), for tests of the error
+ * paths: every entry point then returns COSMA_AMD_ERR_ALLOC with a message
+ * and leaves its handle or plan usable.  Process-wide.  No counterpart in
+ * the reference, whose memory pool prints a hint and re-throws
+ * std::bad_alloc (src/cosma/memory_pool.cpp:65-89,130-150); for tests only.
+ */
+int cosma_amd_test_fail_allocations(int skip, int count);
+
 /* ------------------------------------------------------ pinned host memory */
 /*
  * Allocates `bytes` of page-locked host memory (hipHostMalloc) that the
