@@ -1,5 +1,6 @@
 // Batched INT8 MFMA GEMM with pair lists and residue / FP64 epilogues.
 // See i8gemm.hpp for the contract and the tile layout.
+#include <algorithm>
 #include <type_traits>
 
 #include "i8gemm.hpp"
@@ -30,7 +31,10 @@ __device__ __forceinline__ auto mma(const i32x4 &a, const i32x4 &b, const std::c
         return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
 }
 
-template <int EPI, bool BF16>
+// WIDE (EPI_RES8 with INT8 operands and K > 32768): the residue reduction
+// for sums of any magnitude (a template parameter so that each kernel holds
+// one form: both forms in one kernel made the register allocator spill).
+template <int EPI, bool BF16, bool WIDE = false>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void i8gemm_kernel(I8Problem p, I8Jobs jobs) {
     using acc_t = std::conditional_t<BF16, f32x4, i32x4>;
     // LDS: two stages of 256 x 128-byte rows per operand (128 KB in all).
@@ -131,8 +135,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void
     // hazard recognizer does not look into asm, so the required wait states
     // are inserted by hand: s_nops before the first read (MFMA result ->
     // VALU read) and after the last write (VALU AGPR write -> MFMA SrcC read).
-    const float fmod_m = (float)(jobs.modulus[z] > 0 ? jobs.modulus[z] : 1);
-    const float neg_m = -fmod_m, finv_m = 1.0f / fmod_m;
+    const float finv_m = jobs.fmod[z][0], neg_m = jobs.fmod[z][1];
     auto reduce = [&]() {
         if constexpr (BF16) {
             asm volatile("s_nop 7\n\ts_nop 7\n\ts_nop 7" ::: "memory");
@@ -211,8 +214,6 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void
     }
 
     // ---- epilogue: lane holds rows 4g..4g+3 (contiguous) of column li
-    const int mod = jobs.modulus[z];
-    const float inv_m = 1.0f / (float)(mod > 0 ? mod : 1);
     const double scale = jobs.scale[z];
     if constexpr (EPI == EPI_RES8) {
         // The 256 x 256 int8 residues go through LDS (stage 0 of A, 64 KB,
@@ -222,23 +223,40 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 1))) void
         // [256 c, 256 c + 256) with row r at r ^ (16 (c & 15)): the 4-byte
         // writes of a wave (16 columns x 4 row quads) hit 64 different banks,
         // and each 16-byte chunk stays a contiguous chunk.
+        // sym_mod's quarter-rate multiply and range corrections were most
+        // of the epilogue's instructions.  sym_mod_odd_bits needs sums below
+        // 2^29 in magnitude, i.e. npairs * K <= 32768 (|x| <= npairs * K *
+        // 2^14; the BF16 sums stay below 2^24 for every K), else
+        // sym_mod_odd_bits_wide applies.
+        // Both return a word whose low byte is x's residue also for m = 256
+        // (c16 = 0: y = x & 0xffff, and y - q m = x mod 256 whatever q);
+        // even moduli other than 256 are rejected at launch.
         uint8_t *tile = reinterpret_cast<uint8_t *>(&sa[0][0]);
         __syncthreads();
+        auto stage = [&](auto residue_bits) {
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int rt = wm * 128 + 16 * i + 4 * g;
+            for (int i = 0; i < 8; ++i) {
+                const int rt = wm * 128 + 16 * i + 4 * g;
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const int ct = wn * 128 + 16 * j + li;
-                i32x4 r;
+                for (int j = 0; j < 8; ++j) {
+                    const int ct = wn * 128 + 16 * j + li;
+                    uint32_t b[4];
 #pragma unroll
-                for (int e = 0; e < 4; ++e) r[e] = (int)acc[i][j][e];   // exact integers below 2^24 (BF16)
-                uint32_t w = 0;
-#pragma unroll
-                for (int e = 0; e < 4; ++e) w |= (uint32_t)(uint8_t)(int8_t)sym_mod(r[e], mod, inv_m) << (8 * e);
-                *reinterpret_cast<uint32_t *>(tile + ct * 256 + (rt ^ (16 * (ct & 15)))) = w;
+                    for (int e = 0; e < 4; ++e) b[e] = residue_bits((int)acc[i][j][e]);   // BF16: exact integers
+                    // low bytes of b[0..3] -> one word (v_perm_b32 picks bytes 0-3 of its
+                    // second operand and 4-7 of its first; selector 0x0c gives zero)
+                    const uint32_t lo = __builtin_amdgcn_perm(b[1], b[0], 0x0c0c0400u);
+                    const uint32_t hi = __builtin_amdgcn_perm(b[3], b[2], 0x0c0c0400u);
+                    *reinterpret_cast<uint32_t *>(tile + ct * 256 + (rt ^ (16 * (ct & 15)))) =
+                        __builtin_amdgcn_perm(hi, lo, 0x05040100u);
+                }
             }
-        }
+        };
+        const float c16 = jobs.fmod[z][2];
+        if constexpr (WIDE)
+            stage([&](int x) { return sym_mod_odd_bits_wide(x, c16, finv_m, neg_m); });
+        else
+            stage([&](int x) { return sym_mod_odd_bits(x, c16, finv_m, neg_m); });
         __syncthreads();
         int8_t *out = static_cast<int8_t *>(p.out) + (int64_t)z * p.out_stride;
 #pragma unroll
@@ -306,19 +324,30 @@ int launch_i8gemm(const I8Problem &p, const I8Jobs &jobs, int Z, I8Epilogue epi,
             set_error("i8gemm: job %d has %d pairs", z, jobs.npairs[z]);
             return COSMA_AMD_ERR_INVALID_ARG;
         }
-        if (epi == EPI_RES8 && (jobs.modulus[z] < 2 || jobs.modulus[z] > 256)) {
-            set_error("i8gemm: modulus %d out of range", jobs.modulus[z]);
+        if (epi == EPI_RES8 && (jobs.modulus[z] < 3 || jobs.modulus[z] > 256 ||
+                                (jobs.modulus[z] % 2 == 0 && jobs.modulus[z] != 256))) {
+            set_error("i8gemm: modulus %d is neither 256 nor odd in [3, 255]", jobs.modulus[z]);
             return COSMA_AMD_ERR_INVALID_ARG;
         }
+    }
+    I8Jobs j = jobs;
+    int64_t sum_len = 0;   // longest k sum of a job, in elements (bytes)
+    for (int z = 0; z < Z; ++z) {
+        sum_len = std::max(sum_len, (int64_t)j.npairs[z] * p.K);
+        const int m = j.modulus[z] > 0 ? j.modulus[z] : 1;
+        j.fmod[z][0] = 1.0f / (float)m;
+        j.fmod[z][1] = -(float)m;
+        j.fmod[z][2] = (float)(65536 % m);
     }
     const int64_t tiles = ((p.M + BM - 1) / BM) * ((p.N + BN - 1) / BN);
     const dim3 grid((unsigned)tiles, (unsigned)Z), block(NT);
     switch (epi) {
     case EPI_RES8:
-        if (p.bf16) hipLaunchKernelGGL((i8gemm_kernel<EPI_RES8, true>), grid, block, 0, s, p, jobs);
-        else hipLaunchKernelGGL((i8gemm_kernel<EPI_RES8, false>), grid, block, 0, s, p, jobs);
+        if (p.bf16) hipLaunchKernelGGL((i8gemm_kernel<EPI_RES8, true>), grid, block, 0, s, p, j);
+        else if (sum_len <= 32768) hipLaunchKernelGGL((i8gemm_kernel<EPI_RES8, false>), grid, block, 0, s, p, j);
+        else hipLaunchKernelGGL((i8gemm_kernel<EPI_RES8, false, true>), grid, block, 0, s, p, j);
         break;
-    default: hipLaunchKernelGGL((i8gemm_kernel<EPI_F64ACC, false>), grid, block, 0, s, p, jobs); break;
+    default: hipLaunchKernelGGL((i8gemm_kernel<EPI_F64ACC, false>), grid, block, 0, s, p, j); break;
     }
     CA_HIP(hipGetLastError());
     return COSMA_AMD_OK;

@@ -50,6 +50,9 @@ struct I8Jobs {
     unsigned char pb[I8_MAX_JOBS][I8_MAX_PAIRS];
     int modulus[I8_MAX_JOBS];
     double scale[I8_MAX_JOBS];
+    // set by launch_i8gemm from modulus: 1 / m (correctly rounded), -m and
+    // 2^16 mod m as floats (kernel arguments, so they stay in scalar registers)
+    float fmod[I8_MAX_JOBS][3];
 };
 
 struct I8Problem {
@@ -87,5 +90,30 @@ __device__ __forceinline__ int sym_mod(int x, int m, float inv_m) {
     return r;
 }
 
+// The same residue for odd m and |x| < 2^29 in six full-rate FP32
+// instructions (sym_mod's 32-bit multiply runs at a quarter of the rate); the
+// returned word's low byte is sym_mod(x, m) as a two's-complement byte.
+// x = h 2^16 + l (h = x >> 16, l = x & 0xffff, both exact in FP32) and
+// c16 = 2^16 mod m give y = h c16 + l, congruent to x, with
+// |y| <= 2^13 * 254 + 65535 < 2^21.1: the FMA forming it is exact.
+// q = rint(y * fl(1/m)) is the integer nearest to y / m: the product's error
+// is below |y| / m * 2^-23 < 0.26 / m, and for odd m the quotient y / m lies
+// at least 1 / (2m) from every half-integer.  Hence y - q m is the symmetric
+// residue, |r| <= (m - 1) / 2, and the FMA that forms it from y + 1.5 * 2^23
+// (exact: the sum lies in [2^23, 2^24), where FP32 holds every integer)
+// leaves r + 2^22 in the mantissa, whose low byte is r mod 256.  m = 256
+// needs no arithmetic: the residue is x's low byte.
+constexpr float SYM_MOD_MAGIC = 12582912.0f;   // 1.5 * 2^23
+__device__ __forceinline__ uint32_t sym_mod_odd_bits(int x, float c16, float inv_m, float neg_m) {
+    const float yf = fmaf((float)(x >> 16), c16, (float)(x & 0xffff));
+    const float q = rintf(yf * inv_m);
+    return __float_as_uint(fmaf(q, neg_m, yf + SYM_MOD_MAGIC));
+}
+// Any int x: x - (q m) 2^16, with q the integer nearest to h / m (the same
+// argument; |h| <= 2^15), is congruent to x and below 2^23 in magnitude.
+__device__ __forceinline__ uint32_t sym_mod_odd_bits_wide(int x, float c16, float inv_m, float neg_m) {
+    const float qm = rintf((float)(x >> 16) * inv_m) * neg_m;   // exact: |q m| < 2^16
+    return sym_mod_odd_bits(x + ((int)qm << 16), c16, inv_m, neg_m);
+}
 
 }  // namespace cosma_amd
