@@ -233,53 +233,72 @@ __device__ __forceinline__ void split16(const SplitArgs &a, const double (&v)[16
     }
 }
 
-// op(X) rows contiguous in k (A 'T', B 'N'): thread per (row, 16-k chunk).
+// Order of k inside the residue / slice rows: within each block of 256 k,
+// byte (element) 16 l + j of the row holds k = 32 (j >> 1) + 2 l + (j & 1)
+// for l, j in [0, 16).  Both operands use the same order, so the INT8
+// products over k are unchanged (exact integer sums), and the split kernels
+// read whole 256-byte runs of the FP64 rows per 16 lanes (lane l loads the
+// pairs (k, k + 1) at 32 q + 2 l) while each lane still writes its 16 values
+// as one 16-byte run.
+__device__ __forceinline__ int block_k(int l, int j) { return 32 * (j >> 1) + 2 * l + (j & 1); }
+
+// op(X) rows contiguous in k (A 'T', B 'N'): 16 lanes per (row, 256-k block).
 // Rows in [R, Rpad) and k in [K, Kpad) are written as zeros.
-__global__ void split_contig(SplitArgs a, int64_t Rpad) {
-    const int64_t chunks = a.Kpad / 16;
-    const int64_t total = Rpad * chunks;
+__global__ __launch_bounds__(256) void split_contig(SplitArgs a, int64_t Rpad) {
+    const int64_t blocks = a.Kpad / 256;
+    const int64_t total = Rpad * blocks * 16;
+    const bool vec = ((a.ld & 1) == 0) && ((reinterpret_cast<uintptr_t>(a.x) & 15) == 0);
     for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
          idx += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t r = idx / chunks, k0 = (idx - r * chunks) * 16;
-        double v[16];
+        const int l = (int)(idx & 15);
+        const int64_t rb = idx >> 4;
+        const int64_t r = rb / blocks, k0 = (rb - r * blocks) * 256;
         const bool rin = r < a.R;
-        if (rin && k0 + 16 <= a.K && ((a.ld & 1) == 0) && ((reinterpret_cast<uintptr_t>(a.x) & 15) == 0)) {
-            const f64x2 *p = reinterpret_cast<const f64x2 *>(a.x + k0 + r * a.ld);
+        const double *x = a.x + r * a.ld + k0;
+        double v[16];
+        if (rin && vec && k0 + 256 <= a.K) {
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const f64x2 w = p[j];
-                v[2 * j] = w.x;
-                v[2 * j + 1] = w.y;
+            for (int q = 0; q < 8; ++q) {
+                const f64x2 w = *reinterpret_cast<const f64x2 *>(x + 32 * q + 2 * l);
+                v[2 * q] = w.x;
+                v[2 * q + 1] = w.y;
             }
         } else {
 #pragma unroll
-            for (int j = 0; j < 16; ++j) v[j] = (rin && k0 + j < a.K) ? a.x[k0 + j + r * a.ld] : 0.0;
+            for (int j = 0; j < 16; ++j) {
+                const int kk = block_k(l, j);
+                v[j] = (rin && k0 + kk < a.K) ? x[kk] : 0.0;
+            }
         }
-        split16(a, v, rin ? a.E[r] : 0, a.out + r * a.row_bytes + k0 * a.esize);
+        split16(a, v, rin ? a.E[r] : 0, a.out + r * a.row_bytes + (k0 + 16 * l) * a.esize);
     }
 }
 
-// op(X) rows strided (A 'N', B 'T'): 64 x 64 tiles transposed through LDS
-// (non-temporal loads: 424 -> 406 us at 8192^2; for the contiguous split and
-// the recombination they were slower).
-// blockIdx.x -> 64 rows, blockIdx.y -> 64 k.
+// op(X) rows strided (A 'N', B 'T'): 16 rows x 256 k per workgroup,
+// transposed through LDS (non-temporal loads).  Each k of the tile is one
+// 128-byte run of the 16 rows; each output row leaves as one 256-byte run
+// (16 lanes) per residue plane.
+// blockIdx.x -> 16 rows, blockIdx.y -> 256 k.
 __global__ __launch_bounds__(256) void split_strided(SplitArgs a) {
-    __shared__ double tile[64][65];
-    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    const int64_t r0 = (int64_t)blockIdx.x * 64, k0 = (int64_t)blockIdx.y * 64;
+    __shared__ double tile[256][17];
+    const int t = threadIdx.x;
+    const int64_t r0 = (int64_t)blockIdx.x * 16, k0 = (int64_t)blockIdx.y * 256;
+    {
+        const int row = t & 15, kk = t >> 4;
+        const int64_t r = r0 + row;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int kk = ty + 4 * q;
-        const int64_t r = r0 + tx, k = k0 + kk;
-        tile[kk][tx] = (r < a.R && k < a.K) ? __builtin_nontemporal_load(a.x + r + k * a.ld) : 0.0;
+        for (int q = 0; q < 16; ++q) {
+            const int k = kk + 16 * q;
+            tile[k][row] = (r < a.R && k0 + k < a.K) ? __builtin_nontemporal_load(a.x + r + (k0 + k) * a.ld) : 0.0;
+        }
     }
     __syncthreads();
-    const int row = threadIdx.x >> 2, kc = threadIdx.x & 3;
+    const int l = t & 15, row = t >> 4;
     double v[16];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) v[j] = tile[kc * 16 + j][row];
+    for (int j = 0; j < 16; ++j) v[j] = tile[block_k(l, j)][row];
     const int64_t r = r0 + row;
-    split16(a, v, r < a.R ? a.E[r] : 0, a.out + r * a.row_bytes + (k0 + kc * 16) * a.esize);
+    split16(a, v, r < a.R ? a.E[r] : 0, a.out + r * a.row_bytes + (k0 + 16 * l) * a.esize);
 }
 
 // ------------------------------------------------------ recombination
@@ -887,7 +906,7 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
             if (contig)
                 hipLaunchKernelGGL(split_contig, dim3(grid_for(Rpad * (Kp / 16))), dim3(256), 0, s, sa, Rpad);
             else
-                hipLaunchKernelGGL(split_strided, dim3((unsigned)(Rpad / 64), (unsigned)(Kp / 64)), dim3(256), 0, s, sa);
+                hipLaunchKernelGGL(split_strided, dim3((unsigned)(Rpad / 16), (unsigned)(Kp / 256)), dim3(256), 0, s, sa);
         };
         split(a, lda, m, Mp, a_rows_contig, E, As, a_plane);
         split(b, ldb, n, Np, b_rows_contig, F, Bs, b_plane);
