@@ -44,7 +44,8 @@ PEAK_HBM = 8000.0       # GB/s, MI355X datasheet
 M = N = K = 16384
 ALPHA, BETA = 1.0, 0.5
 METRIC = "FP64-accurate TFLOP/s, 1 GPU: native DGEMM vs split-precision at stated error"
-LEGS = ["device", "host", "tall-skinny", "ozaki", "ozaki-8192", "ozaki-target", "reorder", "batched", "miniapp"]
+LEGS = ["device", "host", "tall-skinny", "ozaki", "ozaki-8192", "ozaki-target", "reorder", "batched", "miniapp",
+        "vendor"]
 
 
 def parse():
@@ -57,7 +58,8 @@ def parse():
                         "leg and the CPU baseline), host (16384^3 from pinned host memory), tall-skinny, ozaki "
                         "(8192^3 phi sweep with the error of every count), ozaki-8192 (the 8192^3 moduli call "
                         "alone at --phi, for profiling), ozaki-target (split precision at the "
-                        "headline shape), reorder, batched, miniapp (2000^3 through cosma::multiply)")
+                        "headline shape), reorder, batched, miniapp (2000^3 through cosma::multiply), vendor "
+                        "(torch's own FP64, INT8 and BF16 GEMMs on this GPU, as reference points)")
     p.add_argument("--phi", type=float, default=0.0, help="phi of the ozaki-8192 leg's operands")
     p.add_argument("--quick", action="store_true", help="headline, split precision and CPU baseline only")
     p.add_argument("--no-split", action="store_true", help="skip the split-precision summary leg")
@@ -679,6 +681,48 @@ def headline(args):
     return out
 
 
+def vendor_reference(steps=5, warmup=2):
+    """The vendor libraries' GEMMs on this GPU through torch (hipBLASLt /
+    rocBLAS), as reference points for the kernels here, not as results: FP64
+    at the headline operation (C <- A^T B + 0.5 C, 16384^3) and at 8192^3 NN,
+    INT8 x INT8 -> INT32 (torch._int_mm, the INT8 product of the split
+    precision) and BF16 at 8192^3 and 16384^3.  Random operands; each entry
+    is the best of two timing runs of `steps` calls."""
+    import torch
+    out = {}
+
+    def run(name, fn, ops, peak, unit):
+        t = min(timed(fn, steps, warmup) for _ in range(2))
+        rate = ops / t / 1e12
+        out[name] = {"ms": round(t * 1e3, 3), "rate": round(rate, 2), "unit": unit, "peak": peak,
+                     "frac": round(rate / peak, 4)}
+        say(f"vendor {name}: {t * 1e3:.3f} ms, {rate:.1f} {unit}")
+
+    for n in (16384, 8192):
+        a = torch.rand(n, n, dtype=torch.float64, device="cuda")
+        b = torch.rand(n, n, dtype=torch.float64, device="cuda")
+        c = torch.rand(n, n, dtype=torch.float64, device="cuda")
+        if n == 16384:
+            run("fp64_addmm_tn_16384", lambda: c.addmm_(a.t(), b, beta=BETA, alpha=ALPHA), 2.0 * n ** 3,
+                PEAK_FP64_MFMA, "TFLOP/s")
+        else:
+            run("fp64_matmul_nn_8192", lambda: torch.mm(a, b, out=c), 2.0 * n ** 3, PEAK_FP64_MFMA, "TFLOP/s")
+        del a, b, c
+        torch.cuda.empty_cache()
+    for n in (8192, 16384):
+        a = torch.randint(-128, 128, (n, n), dtype=torch.int8, device="cuda")
+        b = torch.randint(-128, 128, (n, n), dtype=torch.int8, device="cuda").t()   # k-contiguous columns
+        run(f"int8_int_mm_{n}", lambda: torch._int_mm(a, b), 2.0 * n ** 3, PEAK_INT8, "TOP/s")
+        del a, b
+        x = torch.rand(n, n, dtype=torch.bfloat16, device="cuda")
+        y = torch.rand(n, n, dtype=torch.bfloat16, device="cuda")
+        run(f"bf16_matmul_{n}", lambda: torch.mm(x, y), 2.0 * n ** 3, PEAK_BF16, "TFLOP/s")
+        del x, y
+        torch.cuda.empty_cache()
+    out["libraries"] = f"torch {torch.__version__} (HIP {torch.version.hip})"
+    return out
+
+
 def leg_line(res, steps, warmup):
     """A leg run alone: its record as a full line of the metric."""
     line = {"metric": METRIC, "n_gpus": 1, "steps": steps, "warmup": warmup, "higher_is_better": True,
@@ -728,6 +772,12 @@ def main():
     if w == "miniapp":
         print(json.dumps(leg_line(miniapp(), args.steps, args.warmup)), flush=True)
         return
+    if w == "vendor":
+        res = vendor_reference(args.steps, args.warmup)
+        print(json.dumps(leg_line({"value": res["fp64_addmm_tn_16384"]["rate"], "unit": "TFLOP/s", "config": {
+            "workload": "torch GEMMs on this GPU (reference points)"}, "vendor_reference": res},
+            args.steps, args.warmup)), flush=True)
+        return
 
     out = headline(args)
     if not args.no_split:
@@ -742,6 +792,7 @@ def main():
             "4_split_precision_8192": split_precision(steps=3, warmup=1, sweep=False),
             "5_batched_4096": batched(10, 2),
         }
+        out["vendor_reference"] = vendor_reference()
         out["configs_wall_s"] = round(time.perf_counter() - t0, 1)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n)

@@ -285,7 +285,7 @@ int cosma_amd_handle_last_mode(cosma_amd_handle_t h) { return h ? h->last_mode :
 
 int cosma_amd_dgemm_set_tuning(int tile_config, int group_m) {
     clear_error();
-    CA_CHECK(tile_config >= 0 && tile_config <= 3, "dgemm_set_tuning: tile_config %d not in 0..3", tile_config);
+    CA_CHECK(tile_config >= 0 && tile_config <= 4, "dgemm_set_tuning: tile_config %d not in 0..4", tile_config);
     CA_CHECK(group_m >= 0, "dgemm_set_tuning: negative group_m");
     set_dgemm_tuning(tile_config, group_m);
     return COSMA_AMD_OK;

@@ -423,6 +423,165 @@ __global__ __launch_bounds__(NT) void dgemm_mfma_kernel(
     gemm_tile<BM, BN, TA, TB, VEC>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, vec_c, m0, n0, sm);
 }
 
+// ------------------------------------------ TN kernel with direct LDS loads
+// op(A) = A^T and op(B) = B, i.e. both operands k-contiguous (the headline
+// shape), with M and N multiples of 128, K a multiple of 16 and 16-byte
+// aligned operands with even leading dimensions.  128 x 128 tiles at two
+// workgroups per CU (two waves per SIMD: 128 accumulator registers per
+// lane), so that one workgroup's barrier, fragment reads and load waits
+// overlap the other's MFMAs; the operand tiles go from global memory
+// straight into LDS (global_load_lds_dwordx4: lane l of a wave instruction
+// writes the 16 bytes at the instruction's LDS base + 16 l), with no
+// register staging.  LDS layout, per stage and operand: row r holds its
+// eight k-pairs (128 bytes), pair kp in slot kp ^ (r & 7); the global
+// addresses of the loads apply that permutation, so eight lanes still read
+// one 128-byte row.  The fragments (lane g supplies the pair kp = 4 ks + g
+// of its permuted row), the MFMA order and the epilogue are gemm_tile's,
+// so the results are bit-identical to dgemm_mfma_kernel's.
+constexpr int TN_TILE = 128;
+constexpr int TN_OPER = TN_TILE * KP;    // 16-byte slots of one operand per stage (1024)
+// s_waitcnt immediate (gfx9 fields: vmcnt bits 3:0 and 15:14, expcnt 6:4,
+// lgkmcnt 11:8) waiting for vmcnt = 0 only; OR a count into bits 3:0.
+constexpr int WAIT_VMCNT0 = 0x0F70;
+
+__device__ __forceinline__ void lds_dma16(const double *g, f64x2 *lds) {
+    __builtin_amdgcn_global_load_lds(static_cast<const void *>(g),
+                                     (__attribute__((address_space(3))) void *)lds, 16, 0, 0);
+}
+
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2))) void dgemm_tn_lds_kernel(
+    int64_t M, int64_t N, int64_t K, double alpha,
+    const double *__restrict__ A, int64_t lda,
+    const double *__restrict__ B, int64_t ldb,
+    double beta, double *__restrict__ C, int64_t ldc,
+    int vec_c, int group_m) {
+    __shared__ __attribute__((aligned(16))) f64x2 sm[2][2 * TN_OPER];   // 2 stages x (A, B): 64 KB
+    constexpr int FM = 4, FN = 4;
+    const int64_t tiles_m = M / TN_TILE, tiles_n = N / TN_TILE;
+    const int64_t bid = xcd_remap(blockIdx.x, tiles_m * tiles_n);
+    const int64_t per_group = (int64_t)group_m * tiles_n;
+    const int64_t grp = bid / per_group;
+    const int64_t first_m = grp * group_m;
+    const int64_t gsize = min((int64_t)group_m, tiles_m - first_m);
+    const int64_t in_group = bid - grp * per_group;
+    const int64_t m0 = (first_m + in_group % gsize) * TN_TILE;
+    const int64_t n0 = (in_group / gsize) * TN_TILE;
+
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int g = lane >> 4, li = lane & 15;
+    const int abase = wm * 64 + 4 * (li & 3) + (li >> 2);
+    const int bbase = wn * 64 + li;
+
+    // Loads: slot q * 256 + t of an operand (q = 0..3) is row 32 q + t / 8,
+    // slot t % 8, which holds the pair (t % 8) ^ (row & 7); row & 7 does not
+    // depend on q.
+    const int lrow = t >> 3, lpair = (t & 7) ^ (lrow & 7);
+    const double *pa = A + (m0 + lrow) * lda + 2 * lpair;
+    const double *pb = B + (n0 + lrow) * ldb + 2 * lpair;
+    const int64_t nk = K / BK;
+    auto issue = [&](int64_t kt, int st) {
+        const int64_t k0 = kt * BK;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) lds_dma16(pa + q * 32 * lda + k0, &sm[st][q * 256 + 64 * wave]);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) lds_dma16(pb + q * 32 * ldb + k0, &sm[st][TN_OPER + q * 256 + 64 * wave]);
+    };
+
+    f64x4 acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(0.0, 0.0, f64x4{0.0, 0.0, 0.0, 0.0}, 0, 0, 0);
+    f64x2 af[2][FM], bf[2][FN];
+    auto read_frags = [&](int st, int ks, int h) {
+        const int kp = 4 * ks + g;
+#pragma unroll
+        for (int i = 0; i < FM; ++i) {
+            const int r = abase + 16 * i;
+            af[h][i] = sm[st][r * KP + (kp ^ (r & 7))];
+        }
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+            const int r = bbase + 16 * j;
+            bf[h][j] = sm[st][TN_OPER + r * KP + (kp ^ (r & 7))];
+        }
+    };
+    auto mfma_frags = [&](int h) {
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[h][i].x, bf[h][j].x, acc[i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[h][i].y, bf[h][j].y, acc[i][j], 0, 0, 0);
+    };
+    // Entering iteration kt: stage kt & 1 holds step kt and F[0] its first
+    // sub-step; the loads of step kt + 1 into the other stage are in flight.
+    // Iteration: read sub-step 1 into F[1]; MFMAs of F[0] and F[1]; wait for
+    // this wave's loads (step kt + 1) and barrier (every wave's loads have
+    // landed and every wave has read step kt); refill stage kt & 1 with step
+    // kt + 2 (the last steps reload step nk - 1, never read: no branch); read
+    // the first sub-step of step kt + 1.
+    auto iteration = [&](int64_t kt, int st) {
+        read_frags(st, 1, 1);
+        mfma_frags(0);
+        mfma_frags(1);
+        __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
+        __syncthreads();
+        issue(min(kt + 2, nk - 1), st);
+        read_frags(st ^ 1, 0, 0);
+    };
+    issue(0, 0);
+    issue(min((int64_t)1, nk - 1), 1);
+    __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0 | 8);   // vmcnt(8): step 0 landed
+    __syncthreads();
+    read_frags(0, 0, 0);
+    int64_t kt = 0;
+    for (; kt + 1 < nk; kt += 2) {
+        iteration(kt, 0);
+        iteration(kt + 1, 1);
+    }
+    if (kt < nk) iteration(kt, 0);
+    __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);   // no LDS writes outstanding at exit
+
+    // ---- epilogue (gemm_tile's): lane holds rows 4g..4g+3 of column li
+    const bool beta0 = (beta == 0.0);
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+        const int64_t row = m0 + wm * 64 + 16 * i + 4 * g;
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+            const int64_t col = n0 + wn * 64 + 16 * j + li;
+            double *cp = C + row + col * ldc;
+            const f64x4 r = acc[i][j];
+            if (vec_c) {
+                f64x2 *cv = reinterpret_cast<f64x2 *>(cp);
+                f64x2 lo = {alpha * r[0], alpha * r[1]};
+                f64x2 hi = {alpha * r[2], alpha * r[3]};
+                if (!beta0) {
+                    const f64x2 clo = cv[0], chi = cv[1];
+                    lo.x = fma(beta, clo.x, lo.x); lo.y = fma(beta, clo.y, lo.y);
+                    hi.x = fma(beta, chi.x, hi.x); hi.y = fma(beta, chi.y, hi.y);
+                }
+                cv[0] = lo;
+                cv[1] = hi;
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    double val = alpha * r[e];
+                    if (!beta0) val = fma(beta, cp[e], val);
+                    cp[e] = val;
+                }
+            }
+        }
+    }
+}
+
 // Grouped tiles without LDS ("direct"): one wave per (16 FI) x (16 FJ) C tile
 // (64 x 64 or 32 x 32: up to FI x FJ blocks of 16 x 16 in AGPR accumulators;
 // blocks outside C are skipped),
@@ -700,7 +859,27 @@ int launch_add_scaled(int64_t m, int64_t n, double beta, const double *x, int64_
     return COSMA_AMD_OK;
 }
 
-// Tile configuration: 0 = automatic, 1 = 128x128, 2 = 256x128, 3 = 128x256.
+// The TN direct-LDS kernel's conditions (see dgemm_tn_lds_kernel).
+bool tn_lds_eligible(char ta, char tb, int64_t m, int64_t n, int64_t k, const double *a, int64_t lda,
+                     const double *b, int64_t ldb) {
+    return ta == 'T' && tb == 'N' && m % TN_TILE == 0 && n % TN_TILE == 0 && k % BK == 0 && k > 0 &&
+           aligned16(a) && aligned16(b) && lda % 2 == 0 && ldb % 2 == 0 &&
+           (m / TN_TILE) * (n / TN_TILE) <= 0x7fffffffLL;
+}
+
+int launch_tn_lds(int64_t m, int64_t n, int64_t k, double alpha, const double *a, int64_t lda,
+                  const double *b, int64_t ldb, double beta, double *c, int64_t ldc, int group_m, hipStream_t s) {
+    const int vec_c = aligned16(c) && (ldc % 2 == 0);
+    const dim3 grid((unsigned)((m / TN_TILE) * (n / TN_TILE))), block(NT);
+    hipLaunchKernelGGL(dgemm_tn_lds_kernel, grid, block, 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, vec_c,
+                       group_m);
+    CA_HIP(hipGetLastError());
+    return COSMA_AMD_OK;
+}
+
+// Tile configuration: 0 = automatic, 1 = 128x128, 2 = 256x128, 3 = 128x256,
+// 4 = the TN direct-LDS kernel (128x128, two workgroups per CU) where its
+// conditions hold, else as 0.
 static int g_cfg_override = 0;
 static int g_group_override = 0;
 
@@ -709,6 +888,14 @@ int launch_dgemm_cfg(int cfg, int group_m, char ta, char tb, int64_t m, int64_t 
                      double beta, double *c, int64_t ldc, hipStream_t s) {
     if (m == 0 || n == 0) return COSMA_AMD_OK;
     if (k == 0 || alpha == 0.0) return launch_scale(m, n, beta, c, ldc, s);
+    if (cfg == 4 || cfg == 0) {
+        // TN shapes that fill the GPU take the direct-LDS kernel: bit-identical
+        // results, 1% faster than 128 x 256 tiles at 8192^3 and 16384^3.
+        if (tn_lds_eligible(ta, tb, m, n, k, a, lda, b, ldb) &&
+            (cfg == 4 || (m / TN_TILE) * (n / TN_TILE) >= 1024))
+            return launch_tn_lds(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, group_m > 0 ? group_m : 8, s);
+        cfg = 0;
+    }
     if (cfg == 0) {
         // Large tiles need enough of them to fill 256 CUs; of the two large
         // shapes 128 x 256 measured faster for NN and TN at 8192^3 and 16384^3.
