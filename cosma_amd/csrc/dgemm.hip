@@ -423,69 +423,104 @@ __global__ __launch_bounds__(NT) void dgemm_mfma_kernel(
     gemm_tile<BM, BN, TA, TB, VEC>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, vec_c, m0, n0, sm);
 }
 
-// ------------------------------------------ TN kernel with direct LDS loads
-// op(A) = A^T and op(B) = B, i.e. both operands k-contiguous (the headline
-// shape), with M and N multiples of 128, K a multiple of 16 and 16-byte
-// aligned operands with even leading dimensions.  128 x 128 tiles at two
+// ------------------------------------ GEMM with direct global-to-LDS loads
+// M and N multiples of 128, K a multiple of 16, 16-byte aligned operands with
+// even leading dimensions; any transposes.  128 x 128 tiles at two
 // workgroups per CU (two waves per SIMD: 128 accumulator registers per
 // lane), so that one workgroup's barrier, fragment reads and load waits
 // overlap the other's MFMAs; the operand tiles go from global memory
 // straight into LDS (global_load_lds_dwordx4: lane l of a wave instruction
 // writes the 16 bytes at the instruction's LDS base + 16 l), with no
-// register staging.  LDS layout, per stage and operand: row r holds its
-// eight k-pairs (128 bytes), pair kp in slot kp ^ (r & 7); the global
-// addresses of the loads apply that permutation, so eight lanes still read
-// one 128-byte row.  The fragments (lane g supplies the pair kp = 4 ks + g
-// of its permuted row), the MFMA order and the epilogue are gemm_tile's,
-// so the results are bit-identical to dgemm_mfma_kernel's.
-constexpr int TN_TILE = 128;
-constexpr int TN_OPER = TN_TILE * KP;    // 16-byte slots of one operand per stage (1024)
+// register staging.  LDS layouts per stage and operand:
+//   k-contiguous operand (A 'T', B 'N'): row r holds its eight k-pairs
+//     (128 bytes), pair kp in slot kp ^ (r & 7); the global addresses of the
+//     loads apply that permutation, so eight lanes still read one 128-byte
+//     row.  A fragment is one 16-byte read.
+//   row-contiguous operand (A 'N', B 'T'): k-row k holds the tile's 128 rows
+//     (one 1 KB wave load), k-rows RC_LD doubles apart (the 64-byte pad puts
+//     the four k-rows that one fragment read touches on alternate halves of
+//     the banks).  A fragment is two 8-byte reads (k = 2 kp and 2 kp + 1).
+// The fragments (lane g supplies the pair kp = 4 ks + g of its permuted
+// row), the MFMA order and the epilogue are gemm_tile's, so the results are
+// bit-identical to dgemm_mfma_kernel's.
+constexpr int DL_TILE = 128;
+constexpr int KC_SIZE = DL_TILE * BK;          // doubles of a k-contiguous operand tile
+constexpr int RC_LD = DL_TILE + 8;             // doubles between the k-rows of a row-contiguous tile
+constexpr int RC_SIZE = BK * RC_LD;
 // s_waitcnt immediate (gfx9 fields: vmcnt bits 3:0 and 15:14, expcnt 6:4,
 // lgkmcnt 11:8) waiting for vmcnt = 0 only; OR a count into bits 3:0.
 constexpr int WAIT_VMCNT0 = 0x0F70;
 
-__device__ __forceinline__ void lds_dma16(const double *g, f64x2 *lds) {
+__device__ __forceinline__ void lds_dma16(const double *g, double *lds) {
     __builtin_amdgcn_global_load_lds(static_cast<const void *>(g),
                                      (__attribute__((address_space(3))) void *)lds, 16, 0, 0);
 }
 
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2))) void dgemm_tn_lds_kernel(
+// One operand tile in LDS: the loads of step k0 for wave `wave`, and the
+// fragment of (row r, k-pair kp).
+template <bool KCONTIG>
+struct DlOperand {
+    static constexpr int SIZE = KCONTIG ? KC_SIZE : RC_SIZE;
+    const double *p;   // this thread's load address for k0 = 0
+    int64_t ld;
+    __device__ __forceinline__ DlOperand(const double *x, int64_t ld_, int64_t r0) : ld(ld_) {
+        const int t = threadIdx.x, lane = t & 63;
+        if (KCONTIG) {
+            // slot q * 256 + t is row 32 q + t / 8, slot t % 8, holding the
+            // pair (t % 8) ^ (row & 7); row & 7 does not depend on q
+            const int lrow = t >> 3, lpair = (t & 7) ^ (lrow & 7);
+            p = x + (r0 + lrow) * ld + 2 * lpair;
+        } else {
+            p = x + r0 + 2 * lane;
+        }
+    }
+    __device__ __forceinline__ void issue(int64_t k0, double *tile) const {
+        const int wave = threadIdx.x >> 6;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (KCONTIG) lds_dma16(p + q * 32 * ld + k0, tile + 2 * (q * 256 + 64 * wave));
+            else lds_dma16(p + (k0 + 4 * q + wave) * ld, tile + (4 * q + wave) * RC_LD);
+        }
+    }
+    __device__ __forceinline__ static f64x2 frag(const double *tile, int r, int kp) {
+        if (KCONTIG) return *reinterpret_cast<const f64x2 *>(tile + 2 * (r * KP + (kp ^ (r & 7))));
+        return f64x2{tile[2 * kp * RC_LD + r], tile[(2 * kp + 1) * RC_LD + r]};
+    }
+};
+
+template <bool AK, bool BKC>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2))) void dgemm_dl_kernel(
     int64_t M, int64_t N, int64_t K, double alpha,
     const double *__restrict__ A, int64_t lda,
     const double *__restrict__ B, int64_t ldb,
     double beta, double *__restrict__ C, int64_t ldc,
     int vec_c, int group_m) {
-    __shared__ __attribute__((aligned(16))) f64x2 sm[2][2 * TN_OPER];   // 2 stages x (A, B): 64 KB
+    using OA = DlOperand<AK>;
+    using OB = DlOperand<BKC>;
+    constexpr int STAGE = OA::SIZE + OB::SIZE;
+    __shared__ __attribute__((aligned(16))) double sm[2][STAGE];   // 64-68 KB
     constexpr int FM = 4, FN = 4;
-    const int64_t tiles_m = M / TN_TILE, tiles_n = N / TN_TILE;
+    const int64_t tiles_m = M / DL_TILE, tiles_n = N / DL_TILE;
     const int64_t bid = xcd_remap(blockIdx.x, tiles_m * tiles_n);
     const int64_t per_group = (int64_t)group_m * tiles_n;
     const int64_t grp = bid / per_group;
     const int64_t first_m = grp * group_m;
     const int64_t gsize = min((int64_t)group_m, tiles_m - first_m);
     const int64_t in_group = bid - grp * per_group;
-    const int64_t m0 = (first_m + in_group % gsize) * TN_TILE;
-    const int64_t n0 = (in_group / gsize) * TN_TILE;
+    const int64_t m0 = (first_m + in_group % gsize) * DL_TILE;
+    const int64_t n0 = (in_group / gsize) * DL_TILE;
 
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     const int g = lane >> 4, li = lane & 15;
     const int abase = wm * 64 + 4 * (li & 3) + (li >> 2);
     const int bbase = wn * 64 + li;
-
-    // Loads: slot q * 256 + t of an operand (q = 0..3) is row 32 q + t / 8,
-    // slot t % 8, which holds the pair (t % 8) ^ (row & 7); row & 7 does not
-    // depend on q.
-    const int lrow = t >> 3, lpair = (t & 7) ^ (lrow & 7);
-    const double *pa = A + (m0 + lrow) * lda + 2 * lpair;
-    const double *pb = B + (n0 + lrow) * ldb + 2 * lpair;
+    const OA oa(A, lda, m0);
+    const OB ob(B, ldb, n0);
     const int64_t nk = K / BK;
     auto issue = [&](int64_t kt, int st) {
-        const int64_t k0 = kt * BK;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) lds_dma16(pa + q * 32 * lda + k0, &sm[st][q * 256 + 64 * wave]);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) lds_dma16(pb + q * 32 * ldb + k0, &sm[st][TN_OPER + q * 256 + 64 * wave]);
+        oa.issue(kt * BK, sm[st]);
+        ob.issue(kt * BK, sm[st] + OA::SIZE);
     };
 
     f64x4 acc[FM][FN];
@@ -498,15 +533,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     auto read_frags = [&](int st, int ks, int h) {
         const int kp = 4 * ks + g;
 #pragma unroll
-        for (int i = 0; i < FM; ++i) {
-            const int r = abase + 16 * i;
-            af[h][i] = sm[st][r * KP + (kp ^ (r & 7))];
-        }
+        for (int i = 0; i < FM; ++i) af[h][i] = OA::frag(sm[st], abase + 16 * i, kp);
 #pragma unroll
-        for (int j = 0; j < FN; ++j) {
-            const int r = bbase + 16 * j;
-            bf[h][j] = sm[st][TN_OPER + r * KP + (kp ^ (r & 7))];
-        }
+        for (int j = 0; j < FN; ++j) bf[h][j] = OB::frag(sm[st] + OA::SIZE, bbase + 16 * j, kp);
     };
     auto mfma_frags = [&](int h) {
 #pragma unroll
@@ -526,7 +555,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     // this wave's loads (step kt + 1) and barrier (every wave's loads have
     // landed and every wave has read step kt); refill stage kt & 1 with step
     // kt + 2 (the last steps reload step nk - 1, never read: no branch); read
-    // the first sub-step of step kt + 1.
+    // the first sub-step of step kt + 1.  (Pinning this order with scheduling
+    // barriers measured 1.5-4% slower than the compiler's own placement.)
     auto iteration = [&](int64_t kt, int st) {
         read_frags(st, 1, 1);
         mfma_frags(0);
@@ -859,26 +889,31 @@ int launch_add_scaled(int64_t m, int64_t n, double beta, const double *x, int64_
     return COSMA_AMD_OK;
 }
 
-// The TN direct-LDS kernel's conditions (see dgemm_tn_lds_kernel).
-bool tn_lds_eligible(char ta, char tb, int64_t m, int64_t n, int64_t k, const double *a, int64_t lda,
-                     const double *b, int64_t ldb) {
-    return ta == 'T' && tb == 'N' && m % TN_TILE == 0 && n % TN_TILE == 0 && k % BK == 0 && k > 0 &&
-           aligned16(a) && aligned16(b) && lda % 2 == 0 && ldb % 2 == 0 &&
-           (m / TN_TILE) * (n / TN_TILE) <= 0x7fffffffLL;
+// The direct-LDS kernel's conditions (see dgemm_dl_kernel).
+bool dl_eligible(int64_t m, int64_t n, int64_t k, const double *a, int64_t lda, const double *b, int64_t ldb) {
+    return m % DL_TILE == 0 && n % DL_TILE == 0 && k % BK == 0 && k > 0 && aligned16(a) && aligned16(b) &&
+           lda % 2 == 0 && ldb % 2 == 0 && (m / DL_TILE) * (n / DL_TILE) <= 0x7fffffffLL;
 }
 
-int launch_tn_lds(int64_t m, int64_t n, int64_t k, double alpha, const double *a, int64_t lda,
-                  const double *b, int64_t ldb, double beta, double *c, int64_t ldc, int group_m, hipStream_t s) {
+int launch_dl(char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha, const double *a, int64_t lda,
+              const double *b, int64_t ldb, double beta, double *c, int64_t ldc, int group_m, hipStream_t s) {
     const int vec_c = aligned16(c) && (ldc % 2 == 0);
-    const dim3 grid((unsigned)((m / TN_TILE) * (n / TN_TILE))), block(NT);
-    hipLaunchKernelGGL(dgemm_tn_lds_kernel, grid, block, 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, vec_c,
-                       group_m);
+    const dim3 grid((unsigned)((m / DL_TILE) * (n / DL_TILE))), block(NT);
+#define CA_LAUNCH_DL(X, Y)                                                                                 \
+    hipLaunchKernelGGL((dgemm_dl_kernel<X, Y>), grid, block, 0, s, m, n, k, alpha, a, lda, b, ldb, beta, c, \
+                       ldc, vec_c, group_m)
+    // template flags: operand k-contiguous (A 'T', B 'N')
+    if (ta == 'T' && tb == 'N') CA_LAUNCH_DL(true, true);
+    else if (ta == 'N' && tb == 'N') CA_LAUNCH_DL(false, true);
+    else if (ta == 'N' && tb == 'T') CA_LAUNCH_DL(false, false);
+    else CA_LAUNCH_DL(true, false);
+#undef CA_LAUNCH_DL
     CA_HIP(hipGetLastError());
     return COSMA_AMD_OK;
 }
 
 // Tile configuration: 0 = automatic, 1 = 128x128, 2 = 256x128, 3 = 128x256,
-// 4 = the TN direct-LDS kernel (128x128, two workgroups per CU) where its
+// 4 = the direct-LDS kernel (128x128, two workgroups per CU) where its
 // conditions hold, else as 0.
 static int g_cfg_override = 0;
 static int g_group_override = 0;
@@ -888,20 +923,31 @@ int launch_dgemm_cfg(int cfg, int group_m, char ta, char tb, int64_t m, int64_t 
                      double beta, double *c, int64_t ldc, hipStream_t s) {
     if (m == 0 || n == 0) return COSMA_AMD_OK;
     if (k == 0 || alpha == 0.0) return launch_scale(m, n, beta, c, ldc, s);
+    // Large tiles need enough of them to fill 256 CUs; of the two large
+    // shapes 128 x 256 measured faster for NN and TN at 8192^3 and 16384^3.
+    const int64_t t_large = ((m + 127) / 128) * ((n + 255) / 256);
+    const int auto_cfg = (t_large >= 512) ? 3 : 1;
     if (cfg == 4 || cfg == 0) {
-        // TN shapes that fill the GPU take the direct-LDS kernel: bit-identical
-        // results, 1% faster than 128 x 256 tiles at 8192^3 and 16384^3.
-        if (tn_lds_eligible(ta, tb, m, n, k, a, lda, b, ldb) &&
-            (cfg == 4 || (m / TN_TILE) * (n / TN_TILE) >= 1024))
-            return launch_tn_lds(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, group_m > 0 ? group_m : 8, s);
+        // The direct-LDS kernel (bit-identical results) against 128 x 256
+        // tiles at 8192^3 / 16384^3 measured +5% / +4% (NN), +1.4% / +1.3%
+        // (TN), +6% / +6% (NT), +3% / +3% (TT).  With 512 tiles resident (two
+        // per CU) against 256, its last, partial round can waste more: it is
+        // taken automatically from 1024 tiles on, unless the other choice's
+        // last round is fuller by more than its 4% advantage.
+        bool take = cfg == 4;
+        if (!take && dl_eligible(m, n, k, a, lda, b, ldb)) {
+            const int64_t t = (m / DL_TILE) * (n / DL_TILE);
+            const int64_t t_alt = auto_cfg == 3 ? t_large : ((m + 127) / 128) * ((n + 127) / 128);
+            auto fill = [](int64_t tiles, int64_t slots) {
+                return (double)tiles / (double)(((tiles + slots - 1) / slots) * slots);
+            };
+            take = t >= 1024 && 1.04 * fill(t, 512) >= fill(t_alt, 256);
+        }
+        if (take && dl_eligible(m, n, k, a, lda, b, ldb))
+            return launch_dl(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, group_m > 0 ? group_m : 8, s);
         cfg = 0;
     }
-    if (cfg == 0) {
-        // Large tiles need enough of them to fill 256 CUs; of the two large
-        // shapes 128 x 256 measured faster for NN and TN at 8192^3 and 16384^3.
-        const int64_t t_large = ((m + 127) / 128) * ((n + 255) / 256);
-        cfg = (t_large >= 512) ? 3 : 1;
-    }
+    if (cfg == 0) cfg = auto_cfg;
     switch (cfg) {
     case 1: return launch_cfg<128, 128>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, group_m > 0 ? group_m : 8, s);
     case 2: return launch_cfg<256, 128>(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, group_m > 0 ? group_m : 4, s);

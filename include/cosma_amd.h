@@ -239,10 +239,10 @@ int cosma_amd_handle_last_mode(cosma_amd_handle_t handle);
 /*
  * Selects the DGEMM tile shape for later cosma_amd_dgemm calls (process-wide):
  * tile_config 0 = automatic, 1 = 128x128, 2 = 256x128, 3 = 128x256 (m x n of
- * C per workgroup), 4 = the TN kernel with direct LDS loads (128x128, two
- * workgroups per CU; op(A) = A^T, op(B) = B, m and n multiples of 128, k of
- * 16, 16-byte aligned operands with even leading dimensions; other calls
- * take the automatic choice); group_m = tile-rows walked before moving along n
+ * C per workgroup), 4 = the kernel with direct LDS loads (128x128, two
+ * workgroups per CU; m and n multiples of 128, k of 16, 16-byte aligned
+ * operands with even leading dimensions; other calls take the automatic
+ * choice); group_m = tile-rows walked before moving along n
  * (0 = default for the shape).  No counterpart in the reference (Tiled-MM's
  * tile sizes, COSMA_GPU_MAX_TILE_*, describe host streaming tiles, not
  * kernel tiles); used by the tuner and the benchmarks.

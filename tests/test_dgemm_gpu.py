@@ -88,41 +88,44 @@ def test_every_tile_configuration(handle, cfg):
 
 
 @pytest.mark.parametrize("m,n,k", [(128, 128, 16), (256, 384, 64), (640, 512, 1040), (1024, 1152, 208)])
-def test_tn_direct_lds_kernel_is_bit_identical(handle, m, n, k):
-    """Tile configuration 4 (the TN kernel with direct LDS loads, two
+@pytest.mark.parametrize("ta,tb", [("T", "N"), ("N", "N"), ("N", "T"), ("T", "T")])
+def test_direct_lds_kernel_is_bit_identical(handle, m, n, k, ta, tb):
+    """Tile configuration 4 (direct global-to-LDS loads, 128 x 128 tiles, two
     workgroups per CU) sums every element in the same order as the other
     tile shapes: its results must equal theirs bit for bit, and the oracle's
     within the FP64 bound."""
     import cosma_amd as ca
-    A = oracle.uniform(k, m, 11)          # op(A) = A^T
-    B = oracle.uniform(k, n, 12)
+    A = oracle.uniform(*stored_shape(ta, m, k), 11)
+    B = oracle.uniform(*stored_shape(tb, k, n), 12)
     C0 = oracle.uniform(m, n, 13)
     results = {}
     try:
         for cfg in (4, 3, 1):
             ca.set_dgemm_tuning(cfg, 0)
             for alpha, beta in ((1.0, 0.5), (-0.75, 0.0)):
-                results[(cfg, beta)] = run_gpu(handle, "T", "N", m, n, k, alpha, A, B, beta, C0, nan_c=(beta == 0.0))
+                results[(cfg, beta)] = run_gpu(handle, ta, tb, m, n, k, alpha, A, B, beta, C0, nan_c=(beta == 0.0))
     finally:
         ca.set_dgemm_tuning(0, 0)
     for beta, alpha in ((0.5, 1.0), (0.0, -0.75)):
         assert np.array_equal(results[(4, beta)], results[(3, beta)])
         assert np.array_equal(results[(4, beta)], results[(1, beta)])
-        ref = dd_ref("T", "N", m, n, k, alpha, A, B, beta, C0)
+        ref = dd_ref(ta, tb, m, n, k, alpha, A, B, beta, C0)
         assert oracle.rel_fro(results[(4, beta)], ref) <= gemm_bound(k)
 
 
-def test_tn_direct_lds_kernel_falls_back_off_its_shapes(handle):
+def test_direct_lds_kernel_falls_back_off_its_shapes(handle):
     """Configuration 4 on shapes outside its conditions (m not a multiple of
-    128, k not of 16, NN) takes the automatic kernel and stays correct."""
+    128, k not of 16, an odd leading dimension) takes the automatic kernel
+    and stays correct."""
     import cosma_amd as ca
     try:
         ca.set_dgemm_tuning(4, 0)
-        for (m, n, k, ta, tb) in [(300, 256, 64, "T", "N"), (256, 256, 40, "T", "N"), (256, 256, 64, "N", "N")]:
+        for (m, n, k, ta, tb, pad) in [(300, 256, 64, "T", "N", 0), (256, 256, 40, "N", "N", 0),
+                                       (256, 256, 64, "N", "T", 1)]:
             A = oracle.uniform(*stored_shape(ta, m, k), 1)
             B = oracle.uniform(*stored_shape(tb, k, n), 2)
             C0 = oracle.uniform(m, n, 3)
-            got = run_gpu(handle, ta, tb, m, n, k, 1.0, A, B, 1.0, C0)
+            got = run_gpu(handle, ta, tb, m, n, k, 1.0, A, B, 1.0, C0, lda=A.shape[0] + pad)
             assert oracle.rel_fro(got, dd_ref(ta, tb, m, n, k, 1.0, A, B, 1.0, C0)) <= gemm_bound(k)
     finally:
         ca.set_dgemm_tuning(0, 0)
