@@ -61,12 +61,13 @@ __global__ void clear_u64(unsigned long long *p, int64_t n) {
 // dimension in registers, and reduces across its lanes with shuffles.
 // CONTIG (element (r, k) at x[k + r * ld]): lane = k, wave = 64 vectors, one
 // wave max and sum per vector and STATS_KC * 64 k; otherwise (x[r + k * ld]):
-// lane = r, wave = 64 k, one wave sum per k.  Workgroups cover STATS_KC * 64
-// (CONTIG) or 64 lanes x STATS_SPAN of the other dimension (4 waves x
-// STATS_SPAN / 4).
+// lane = r within STATS_RCH chunks of 64 rows, wave = 64 k, one wave sum per
+// k.  Workgroups cover STATS_KC * 64 (CONTIG) or STATS_RCH * 64 lanes x
+// STATS_SPAN of the other dimension (4 waves x STATS_SPAN / 4).
 // The contiguous form reads with non-temporal loads (8192^2: 140 -> 128 us).
 constexpr int STATS_SPAN = 256;
 constexpr int STATS_KC = 8;   // 64-wide k chunks per workgroup (contiguous vectors)
+constexpr int STATS_RCH = 4;  // 64-row chunks per wave (strided vectors)
 
 template <bool CONTIG>
 __global__ __launch_bounds__(256) void stats_kernel(const double *__restrict__ x, int64_t ld, int64_t R, int64_t K,
@@ -141,27 +142,45 @@ __global__ __launch_bounds__(256) void stats_kernel(const double *__restrict__ x
         for (int c = 0; c < STATS_KC; ++c)
             if (kbase + 64 * c < K) atomicAdd(&ksq[kbase + 64 * c], ss[c]);
     } else {
-        const int64_t r = (int64_t)blockIdx.x * 64 + lane;
+        // A wave covers STATS_RCH chunks of 64 rows (lane = row within a
+        // chunk) for PER_WAVE k: the per-k sums of squares accumulate over
+        // the chunks in the lane before one cross-lane reduction, and the
+        // STATS_RCH * U loads of a group are all in flight together.
+        const int64_t rbase = (int64_t)blockIdx.x * 64 * STATS_RCH + lane;
         const int64_t kb = (int64_t)blockIdx.y * STATS_SPAN + wave * PER_WAVE;
-        const bool rin = r < R;
-        unsigned long long best = 0;
-        double rs = 0.0, ra = 0.0;
-        for (int q0 = 0; q0 < PER_WAVE; q0 += U) {
-            double v[U];
+        unsigned long long best[STATS_RCH];
+        double rs[STATS_RCH], ra[STATS_RCH];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int64_t k = kb + q0 + u;
-                v[u] = (rin && k < K) ? x[r + k * ld] : 0.0;
+        for (int c = 0; c < STATS_RCH; ++c) {
+            best[c] = 0;
+            rs[c] = 0.0;
+            ra[c] = 0.0;
+        }
+        for (int q0 = 0; q0 < PER_WAVE; q0 += U) {
+            double v[STATS_RCH][U];
+#pragma unroll
+            for (int c = 0; c < STATS_RCH; ++c) {
+                const int64_t r = rbase + 64 * c;
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int64_t k = kb + q0 + u;
+                    v[c][u] = (r < R && k < K) ? x[r + k * ld] : 0.0;
+                }
             }
             double sq[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const unsigned long long b = __double_as_longlong(fabs(v[u]));
-                best = b > best ? b : best;
-                sq[u] = v[u] * v[u];
-                rs += sq[u];
-                ra += fabs(v[u]);
-            }
+            for (int u = 0; u < U; ++u) sq[u] = 0.0;
+#pragma unroll
+            for (int c = 0; c < STATS_RCH; ++c)
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const unsigned long long b = __double_as_longlong(fabs(v[c][u]));
+                    best[c] = b > best[c] ? b : best[c];
+                    const double q = v[c][u] * v[c][u];
+                    sq[u] += q;
+                    rs[c] += q;
+                    ra[c] += fabs(v[c][u]);
+                }
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) {
 #pragma unroll
@@ -174,10 +193,14 @@ __global__ __launch_bounds__(256) void stats_kernel(const double *__restrict__ x
                 atomicAdd(&ksq[kb + q0 + lane], mine);
             }
         }
-        if (rin) {
-            atomicMax(&maxbits[r], best);
-            atomicAdd(&rsq[r], rs);
-            atomicAdd(&rabs[r], ra);
+#pragma unroll
+        for (int c = 0; c < STATS_RCH; ++c) {
+            const int64_t r = rbase + 64 * c;
+            if (r < R) {
+                atomicMax(&maxbits[r], best[c]);
+                atomicAdd(&rsq[r], rs[c]);
+                atomicAdd(&rabs[r], ra[c]);
+            }
         }
     }
 }
@@ -730,7 +753,7 @@ int ozaki_dgemm(cosma_amd_handle_t h, char ta, char tb, int64_t m, int64_t n, in
     const bool a_rows_contig = ta == 'T', b_rows_contig = tb == 'N';   // op rows contiguous in k
     // grid: x over the contiguous dimension in 64s, y over the other in STATS_SPAN
     auto stats_grid = [](int64_t R, int64_t K, bool contig) {
-        const int64_t fast = contig ? K : R, slow = contig ? R : K, span = contig ? 64 * STATS_KC : 64;
+        const int64_t fast = contig ? K : R, slow = contig ? R : K, span = 64 * (contig ? STATS_KC : STATS_RCH);
         return dim3((unsigned)((fast + span - 1) / span), (unsigned)((slow + STATS_SPAN - 1) / STATS_SPAN));
     };
     if (a_rows_contig) hipLaunchKernelGGL(stats_kernel<true>, stats_grid(m, k, true), dim3(256), 0, s, a, lda, m, k, maxA, ksqA, rsqA, rabsA);
