@@ -564,19 +564,18 @@ const double *pow2_table() {
 }
 
 // std::ldexp(x, e) without its per-call cost (tens of thousands of calls per
-// product made the host statistics take ~0.4 ms at 8192^3): one
-// multiplication by an exact power of two when that power and the product
-// are normal numbers, where the product is exact and equal to ldexp's;
-// ldexp itself otherwise (subnormal, zero, infinite or NaN results).
-// (P2 = pow2_table(), fetched once outside the loops.)
+// product made the host statistics take ~0.4 ms at 8192^3): while 2^e is a
+// normal double, x * 2^e is the correctly rounded value of x 2^e (a single
+// rounding, also for subnormal or overflowing results), which is what ldexp
+// returns; ldexp itself otherwise.  (P2 = pow2_table(), fetched once outside
+// the loops.)
 inline double scale2(const double *P2, double x, int e) {
-    if (e > -1022 && e < 1024) {
-        const double r = x * P2[e];
-        const double a = std::fabs(r);
-        if (a >= DBL_MIN && a <= DBL_MAX) return r;
-    }
-    return std::ldexp(x, e);
+    return (e >= -1022 && e <= 1023) ? x * P2[e] : std::ldexp(x, e);
 }
+
+// 2^e for e <= 0: the table holds ldexp(1, e) down to -1100, and every
+// smaller power rounds to zero.
+inline double pow2_nonpos(const double *P2, int e) { return e >= -1100 ? P2[e] : 0.0; }
 
 // A read-only view of n host values.
 template <typename T>
@@ -604,25 +603,44 @@ ErrModel error_model(View<int> E, View<int> F, View<double> ksqA, View<double> k
     em.fb = fb;
     const double *P2 = pow2_table();
     double SA = 0.0, SB = 0.0, NA = 0.0, NB = 0.0, D = 0.0, LA = 0.0, LB = 0.0;
-    for (int64_t i = 0; i < E.size(); ++i) {
-        if (E[i] == ZERO_EXPONENT) continue;
-        SA += scale2(P2, 1.0, 2 * (E[i] - ea));
-        const double a = scale2(P2, l1A[i], -ea);
-        LA += a * a;
-    }
-    for (int64_t j = 0; j < F.size(); ++j) {
-        if (F[j] == ZERO_EXPONENT) continue;
-        SB += scale2(P2, 1.0, 2 * (F[j] - fb));
-        const double b = scale2(P2, l1B[j], -fb);
-        LB += b * b;
-    }
+    // the same sums as with scale2 per term: the loop-invariant powers are
+    // hoisted (a product by an exact normal power of two is scale2's result)
+    auto vec_sums = [P2](View<int> ex, View<double> l1, int top, double &S_out, double &L_out) {
+        const bool normal = -top >= -1022 && -top <= 1023;
+        const double s = normal ? P2[-top] : 0.0;
+        const int *e = ex.p;
+        const double *x = l1.p;
+        double S = 0.0, L = 0.0;
+        for (int64_t i = 0; i < ex.size(); ++i) {
+            if (e[i] == ZERO_EXPONENT) continue;
+            S += pow2_nonpos(P2, 2 * (e[i] - top));
+            const double a = normal ? x[i] * s : std::ldexp(x[i], -top);
+            L += a * a;
+        }
+        S_out = S;
+        L_out = L;
+    };
+    vec_sums(E, l1A, ea, SA, LA);
+    vec_sums(F, l1B, fb, SB, LB);
     bool finite = true;
-    for (int64_t l = 0; l < ksqA.size(); ++l) {
-        if (!std::isfinite(ksqA[l]) || !std::isfinite(ksqB[l])) finite = false;
-        const double a = scale2(P2, ksqA[l], -2 * ea), b = scale2(P2, ksqB[l], -2 * fb);
-        NA += a;
-        NB += b;
-        D += a * b;
+    {
+        const bool na = -2 * ea >= -1022 && -2 * ea <= 1023, nb = -2 * fb >= -1022 && -2 * fb <= 1023;
+        const double sa = na ? P2[-2 * ea] : 0.0, sb = nb ? P2[-2 * fb] : 0.0;
+        const double *qa = ksqA.p, *qb = ksqB.p;
+        double na_ = 0.0, nb_ = 0.0, d_ = 0.0;
+        int nonfin = 0;
+        for (int64_t l = 0; l < ksqA.size(); ++l) {
+            nonfin |= !std::isfinite(qa[l]) | !std::isfinite(qb[l]);
+            const double a = na ? qa[l] * sa : std::ldexp(qa[l], -2 * ea);
+            const double b = nb ? qb[l] * sb : std::ldexp(qb[l], -2 * fb);
+            na_ += a;
+            nb_ += b;
+            d_ += a * b;
+        }
+        NA = na_;
+        NB = nb_;
+        D = d_;
+        finite = !nonfin;
     }
     // the 1-norms are FP64 sums of |x|: a relative margin covers their rounding
     const double margin = 1.0 + 1e-12;
