@@ -113,6 +113,28 @@ def test_direct_lds_kernel_is_bit_identical(handle, m, n, k, ta, tb):
         assert oracle.rel_fro(results[(4, beta)], ref) <= gemm_bound(k)
 
 
+@pytest.mark.parametrize("ta,tb", [("T", "N"), ("N", "N"), ("N", "T"), ("T", "T")])
+def test_direct_lds_kernel_padded_leading_dimensions(handle, ta, tb):
+    """Even padded leading dimensions keep the direct-LDS kernel (16-byte
+    aligned rows and k-rows) and must give the same bits as the other tile
+    shapes."""
+    import cosma_amd as ca
+    m, n, k = 256, 384, 96
+    A = oracle.uniform(*stored_shape(ta, m, k), 21)
+    B = oracle.uniform(*stored_shape(tb, k, n), 22)
+    C0 = oracle.uniform(m, n, 23)
+    lda, ldb, ldc = A.shape[0] + 6, B.shape[0] + 2, m + 4
+    out = {}
+    try:
+        for cfg in (4, 1):
+            ca.set_dgemm_tuning(cfg, 0)
+            out[cfg] = run_gpu(handle, ta, tb, m, n, k, 1.25, A, B, -0.5, C0, lda=lda, ldb=ldb, ldc=ldc)
+    finally:
+        ca.set_dgemm_tuning(0, 0)
+    assert np.array_equal(out[4], out[1])
+    assert oracle.rel_fro(out[4], dd_ref(ta, tb, m, n, k, 1.25, A, B, -0.5, C0)) <= gemm_bound(k)
+
+
 def test_direct_lds_kernel_falls_back_off_its_shapes(handle):
     """Configuration 4 on shapes outside its conditions (m not a multiple of
     128, k not of 16, an odd leading dimension) takes the automatic kernel
