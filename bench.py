@@ -279,6 +279,17 @@ def timed(fn, steps, warmup):
     return (time.perf_counter() - t0) / steps
 
 
+def full_diff(C, C_fp64):
+    """Relative Frobenius difference of the whole matrix C against the FP64
+    GPU result (itself parity-tested against the oracle), on the device: it
+    bounds every element, where the oracle comparison samples rows."""
+    import torch
+    d = torch.linalg.vector_norm(C - C_fp64).item()
+    r = torch.linalg.vector_norm(C_fp64).item()
+    ok = bool(torch.isfinite(C).all().item())
+    return {"rel_fro_whole_matrix_vs_fp64_gpu": d / r if r > 0 else d, "all_finite": ok}
+
+
 def ozaki_entry(info, t, t_plain, flops, err, n):
     """Per-method record of a split-precision leg, with the product kernel's
     roofline (INT8 or BF16 matrix peak; 2 n^3 ops per product)."""
@@ -302,7 +313,8 @@ def split_precision(phis=(0.0, 1.0, 4.0), steps=5, warmup=1, n=8192, rows=(0, 40
     (moduli at the default 1e-14 tolerance with INT8 and with BF16 residues,
     slices at the default tolerance), inputs a = (u - 0.5) exp(phi z).  Errors
     are relative Frobenius errors of sampled rows against the double-double
-    oracle.  With `sweep`, also the error and time of every count."""
+    oracle, with the whole-matrix difference against the FP64 GPU result.
+    With `sweep`, also the error and time of every count."""
     import torch
     import cosma_amd as ca
     h = ca.Handle(0)
@@ -321,6 +333,7 @@ def split_precision(phis=(0.0, 1.0, 4.0), steps=5, warmup=1, n=8192, rows=(0, 40
         t_plain = timed(plain, steps, warmup)
         entry = {"phi": phi, "fp64": {"tflops": round(flops / t_plain / 1e12, 3), "ms": round(t_plain * 1e3, 3),
                                       "rel_err": rows_error(C, rows, ref)}}
+        C_fp64 = C.clone()
         for name, method in (("moduli", ca.OZAKI_MODULI), ("moduli_bf16", ca.OZAKI_MODULI_BF16),
                              ("slices", ca.OZAKI_SLICES)):
             if name not in methods:
@@ -332,6 +345,8 @@ def split_precision(phis=(0.0, 1.0, 4.0), steps=5, warmup=1, n=8192, rows=(0, 40
                                           C.data_ptr(), n, method=method, tolerance=1e-14))
             t = timed(oz, steps, warmup)
             entry[name] = ozaki_entry(info, t, t_plain, flops, rows_error(C, rows, ref), n)
+            entry[name].update(full_diff(C, C_fp64))
+        del C_fp64
         if sweep:
             # error against the double-double oracle for each count: moduli 8..20
             # and slices 4..8 (one warm-up and one timed call each)
@@ -361,7 +376,7 @@ def split_precision_target(steps=3, warmup=1, n=16384, rows=(0, 8193)):
     alpha = 1, uniform [-1, 1) inputs, beta = 0 (so that C needs no restoring
     between timed calls): plain FP64 MFMA DGEMM vs the moduli method at the
     default 1e-14 tolerance, errors of sampled rows against the double-double
-    oracle."""
+    oracle and the whole-matrix difference against the FP64 GPU result."""
     import torch
     import cosma_amd as ca
     say("split precision at the headline shape")
@@ -377,6 +392,7 @@ def split_precision_target(steps=3, warmup=1, n=16384, rows=(0, 8193)):
     t_plain = timed(lambda: h.dgemm("T", "N", n, n, n, 1.0, A.data_ptr(), n, B.data_ptr(), n, 0.0, C.data_ptr(), n),
                     steps, warmup)
     e_plain = rows_error(C, rows, ref)
+    C_fp64 = C.clone()
     info = {}
 
     def oz():
@@ -386,8 +402,9 @@ def split_precision_target(steps=3, warmup=1, n=16384, rows=(0, 8193)):
     out = {"workload": "C=A^T*B m=n=k=16384 alpha=1 beta=0, uniform[-1,1) inputs",
            "fp64": {"tflops": round(flops / t_plain / 1e12, 3), "ms": round(t_plain * 1e3, 3), "rel_err": e_plain},
            "moduli": ozaki_entry(info, t_oz, t_plain, flops, rows_error(C, rows, ref), n)}
+    out["moduli"].update(full_diff(C, C_fp64))
     h.close()
-    del A, B, C
+    del A, B, C, C_fp64
     torch.cuda.empty_cache()
     return out
 
