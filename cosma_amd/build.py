@@ -38,6 +38,11 @@ COMMON = [
 
 
 HOST = ["-std=c++17", "-O2", "-fPIC", "-Wall", "-I", os.path.join(ROOT, "include"), "-I", FRONT]
+# The C++ interface takes MPI_Comm: built against <mpi.h> from
+# COSMA_AMD_MPI_INCLUDE when set (e.g. Open MPI, whose handles are pointers),
+# else against the MPICH ABI's integer handles.
+if os.environ.get("COSMA_AMD_MPI_INCLUDE"):
+    HOST += ["-I", os.environ["COSMA_AMD_MPI_INCLUDE"]]
 
 
 HOSTSRC = os.path.join(CSRC, "host")

@@ -12,9 +12,12 @@ namespace detail {
 void *find_symbol(const char *name) { return dlsym(RTLD_DEFAULT, name); }
 
 int comm_size(MPI_Comm comm) {
+#if !defined(OPEN_MPI)
     // Open MPI's handles are pointers, not the integer handles this
-    // interface was built with: nothing can be asked then.
+    // interface was built with: nothing can be asked then (build with
+    // Open MPI's mpi.h on the include path to ask it).
     if (find_symbol("ompi_mpi_comm_world")) return 1;
+#endif
     using initialized_fn = int (*)(int *);
     using size_fn = int (*)(MPI_Comm, int *);
     auto initialized = reinterpret_cast<initialized_fn>(find_symbol("MPI_Initialized"));
