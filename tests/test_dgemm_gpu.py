@@ -114,16 +114,17 @@ def test_direct_lds_kernel_is_bit_identical(handle, m, n, k, ta, tb):
 
 
 @pytest.mark.parametrize("ta,tb", [("T", "N"), ("N", "N"), ("N", "T"), ("T", "T")])
-def test_direct_lds_kernel_padded_leading_dimensions(handle, ta, tb):
-    """Even padded leading dimensions keep the direct-LDS kernel (16-byte
-    aligned rows and k-rows) and must give the same bits as the other tile
-    shapes."""
+@pytest.mark.parametrize("cpad", [4, 3])
+def test_direct_lds_kernel_padded_leading_dimensions(handle, ta, tb, cpad):
+    """Even padded leading dimensions of A and B keep the direct-LDS kernel
+    (16-byte aligned rows and k-rows) and must give the same bits as the
+    other tile shapes; an odd ldc takes its 8-byte epilogue."""
     import cosma_amd as ca
     m, n, k = 256, 384, 96
     A = oracle.uniform(*stored_shape(ta, m, k), 21)
     B = oracle.uniform(*stored_shape(tb, k, n), 22)
     C0 = oracle.uniform(m, n, 23)
-    lda, ldb, ldc = A.shape[0] + 6, B.shape[0] + 2, m + 4
+    lda, ldb, ldc = A.shape[0] + 6, B.shape[0] + 2, m + cpad
     out = {}
     try:
         for cfg in (4, 1):
