@@ -549,22 +549,48 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2))) void
             for (int j = 0; j < FN; ++j)
                 acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[h][i].y, bf[h][j].y, acc[i][j], 0, 0, 0);
     };
+    // MFMAs [from, to) of sub-step h in mfma_frags' order (.x block, then .y)
+    auto mfma_range = [&](int h, int from, int to) {
+#pragma unroll
+        for (int q = 0; q < 2 * FM * FN; ++q) {
+            if (q < from || q >= to) continue;
+            const int i = (q % (FM * FN)) / FN, j = q % FN;
+            if (q < FM * FN) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[h][i].x, bf[h][j].x, acc[i][j], 0, 0, 0);
+            else acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[h][i].y, bf[h][j].y, acc[i][j], 0, 0, 0);
+        }
+    };
     // Entering iteration kt: stage kt & 1 holds step kt and F[0] its first
     // sub-step; the loads of step kt + 1 into the other stage are in flight.
-    // Iteration: read sub-step 1 into F[1]; MFMAs of F[0] and F[1]; wait for
-    // this wave's loads (step kt + 1) and barrier (every wave's loads have
-    // landed and every wave has read step kt); refill stage kt & 1 with step
-    // kt + 2 (the last steps reload step nk - 1, never read: no branch); read
-    // the first sub-step of step kt + 1.  (Pinning this order with scheduling
-    // barriers measured 1.5-4% slower than the compiler's own placement.)
+    // Iteration: read sub-step 1 into F[1]; MFMAs of F[0]; wait for this
+    // wave's loads (step kt + 1) and barrier (every wave's loads have landed
+    // and every wave has read step kt); refill stage kt & 1 with step kt + 2
+    // (the last steps reload step nk - 1, never read: no branch); read the
+    // first sub-step of step kt + 1 into F[0]; MFMAs of F[1].  The MFMAs of
+    // F[1] after the barrier cover the latency of the F[0] reads, and those
+    // of F[0] before it the latency of the F[1] reads, so the barrier's own
+    // LDS wait finds nothing outstanding.  The scheduling barriers hold that
+    // order: left alone the compiler moved the MFMAs past the barrier and
+    // the fragment reads up to it, so that the barrier waited for the reads
+    // with the MFMA pipe idle.  With both operands k-contiguous (A 'T',
+    // B 'N') the F[1] reads measured fastest issued as one batch ahead of
+    // all F[0] MFMAs (+1.5% against the compiler's order at 8192^3 and
+    // 16384^3), for the other layouts behind the first four F[0] MFMAs
+    // (+0.2-0.5%; the batch form there 0.3% slower).
     auto iteration = [&](int64_t kt, int st) {
+        constexpr int HEAD = (AK && BKC) ? 0 : 4;
+        mfma_range(0, 0, HEAD);
+        __builtin_amdgcn_sched_barrier(0);
         read_frags(st, 1, 1);
-        mfma_frags(0);
-        mfma_frags(1);
+        __builtin_amdgcn_sched_barrier(0);
+        mfma_range(0, HEAD, 2 * FM * FN);
+        __builtin_amdgcn_sched_barrier(0);
         __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
         __syncthreads();
         issue(min(kt + 2, nk - 1), st);
         read_frags(st ^ 1, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
+        mfma_frags(1);
+        __builtin_amdgcn_sched_barrier(0);
     };
     issue(0, 0);
     issue(min((int64_t)1, nk - 1), 1);
@@ -929,7 +955,7 @@ int launch_dgemm_cfg(int cfg, int group_m, char ta, char tb, int64_t m, int64_t 
     const int auto_cfg = (t_large >= 512) ? 3 : 1;
     if (cfg == 4 || cfg == 0) {
         // The direct-LDS kernel (bit-identical results) against 128 x 256
-        // tiles at 8192^3 / 16384^3 measured +5% / +4% (NN), +1.4% / +1.3%
+        // tiles at 8192^3 / 16384^3 measured +5% / +4% (NN), +3% / +2.7%
         // (TN), +6% / +6% (NT), +3% / +3% (TT).  With 512 tiles resident (two
         // per CU) against 256, its last, partial round can waste more: it is
         // taken automatically from 1024 tiles on, unless the other choice's
