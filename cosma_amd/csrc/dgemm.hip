@@ -537,19 +537,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 #pragma unroll
         for (int j = 0; j < FN; ++j) bf[h][j] = OB::frag(sm[st] + OA::SIZE, bbase + 16 * j, kp);
     };
-    auto mfma_frags = [&](int h) {
-#pragma unroll
-        for (int i = 0; i < FM; ++i)
-#pragma unroll
-            for (int j = 0; j < FN; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[h][i].x, bf[h][j].x, acc[i][j], 0, 0, 0);
-#pragma unroll
-        for (int i = 0; i < FM; ++i)
-#pragma unroll
-            for (int j = 0; j < FN; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[h][i].y, bf[h][j].y, acc[i][j], 0, 0, 0);
-    };
-    // MFMAs [from, to) of sub-step h in mfma_frags' order (.x block, then .y)
+    // MFMAs [from, to) of sub-step h, in gemm_tile's order: the .x block
+    // (k = 2 kp), then the .y block (k = 2 kp + 1), each row-major in (i, j)
     auto mfma_range = [&](int h, int from, int to) {
 #pragma unroll
         for (int q = 0; q < 2 * FM * FN; ++q) {
@@ -589,7 +578,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2))) void
         issue(min(kt + 2, nk - 1), st);
         read_frags(st ^ 1, 0, 0);
         __builtin_amdgcn_sched_barrier(0);
-        mfma_frags(1);
+        mfma_range(1, 0, 2 * FM * FN);
         __builtin_amdgcn_sched_barrier(0);
     };
     issue(0, 0);
